@@ -1,0 +1,147 @@
+"""Benchmark: chem cells/s per flow step on one GPU (plain FP64 integrator).
+
+Workload (the 1,048,576-cell H2-air configuration): 9 species / 19 reactions,
+cells seeded with T ~ U(900, 2500) K, p ~ U(0.5, 5) atm, H2/air mixture
+fraction Z ~ U(0, 1) plus 1e-6 radical seeds.  One step = one flow step of
+the chemistry path on device-resident SoA fields: restore (T, Y, subDt) from
+pristine copies, chemistrySource::solveUpdate over dt = 1e-6 s with the
+ImplicitEuler integrator (rtol 1e-6, atol 1e-10), then mixture viscosity,
+conductivity and mixture-averaged D_k at the new state.
+
+Prints ONE JSON line (see the repository's measurement notes for the fields).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "chem cells/s per flow step on 1 GPU; surrogate vs plain at ≤1e-3 rel err"
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix) peak
+
+
+def cpu_baseline(mech, opts, states, n_sample, threads):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from oracle import Oracle
+
+    orc = Oracle(mech)
+    sl = slice(0, n_sample)
+    T, p, rho = (np.ascontiguousarray(states[k][sl]) for k in ("T", "p", "rho"))
+    Y = np.ascontiguousarray(states["Y"][:, sl])
+    dt = np.full(n_sample, 1e-6)
+    t0 = time.perf_counter()
+    orc.solve_update(opts, T, p, rho, Y, dt, 1.0, nthreads=threads)
+    el = time.perf_counter() - t0
+    return {"value": n_sample / el, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"first {n_sample} cells of the same seeded workload, "
+                      f"{threads} host threads, {el:.2f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=1 << 20)
+    ap.add_argument("--cpu-sample", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
+              "the chemistry path shards by contiguous cell ranges with no communication, "
+              "see the design notes", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource, Mechanism
+    from openhurricane_amd.flops import integrator_flops, work_flops
+    from openhurricane_amd.sampler import h2air_states
+
+    dev = torch.device("cuda:0")
+    mech = Mechanism.h2air()
+    tab = mech.tables()
+    n = args.cells
+    st = h2air_states(mech.species, tab["W"], n, seed=2024)
+    opts = _lib.default_opts(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6,
+                             device_pointers=1)
+    cs = ChemistrySource(mech, device=0, opts=opts)
+    stream = torch.cuda.current_stream(dev)
+    cs.set_stream(stream.cuda_stream)
+    g = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in st.items()}
+    T0, Y0 = g["T"].clone(), g["Y"].clone()
+    T, Y = g["T"].clone(), g["Y"].clone()
+    p, rho = g["p"], g["rho"]
+    dt = torch.full((n,), 1e-6, dtype=torch.float64, device=dev)
+    sub0 = torch.zeros(n, dtype=torch.float64, device=dev)
+    sub = sub0.clone()
+
+    def step(ev=None):
+        T.copy_(T0)
+        Y.copy_(Y0)
+        sub.copy_(sub0)
+        if ev:
+            ev[0].record(stream)
+        cs.solve_update(T, p, rho, Y, dt, 1.0, subDt=sub, stats=False)
+        if ev:
+            ev[1].record(stream)
+        props = cs.mix_props(T, p, Y, mu=True, kappa=True, Dk=True, cp=False, ha=False)
+        return props
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(evs[i])
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms = el * 1e3 / args.steps
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+
+    # counters of one advance (deterministic: same inputs -> same work)
+    T.copy_(T0)
+    Y.copy_(Y0)
+    sub.copy_(sub0)
+    _, stats = cs.solve_update(T, p, rho, Y, dt, 1.0, subDt=sub, stats=True)
+    fl = work_flops(tab, stats)
+    achieved = fl / (kern_ms * 1e-3) / 1e12
+    roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "kernel": "ie_lane_kernel<Mech_h2air>", "kernel_ms": kern_ms,
+            "flops_per_launch": fl, "newton_iters": stats["newton_iters"],
+            "steps": stats["steps"], "flops_per_newton": integrator_flops(tab)["F_newton"]}
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        cpu = cpu_baseline(mech, _lib.default_opts(atol=1e-10, rtol=1e-6, atol_inc=1e-10,
+                                                   rtol_inc=1e-6),
+                           st, args.cpu_sample, threads)
+    line = {
+        "metric": METRIC, "value": n / (ms * 1e-3), "unit": "cells/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "H2-air 9sp/19rxn, 1,048,576 cells, dt 1e-6 s, plain FP64 "
+                               "ImplicitEuler (rtol 1e-6, atol 1e-10) + mu/kappa/D_k",
+                   "cells": n, "mechanism": "h2air", "integrator": "ImplicitEuler",
+                   "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

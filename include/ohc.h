@@ -1,0 +1,235 @@
+/*
+ * ohc.h -- C-ABI of the MI355X per-cell chemistry engine (libohc.so).
+ *
+ * This is the drop-in boundary for OpenHurricane's per-cell chemistry hot path.
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (OHC_OK = 0) and never aborts the process; the text of the last
+ * error is read with ohc_last_error(ctx) (ctx may be NULL for errors raised
+ * before a context exists).
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * OpenHurricane source tree):
+ *   ohc_mech_load / ohc_mech_from_tables
+ *       mixture::cuParsingReaction / cuAddThermoToCUDA + cuChemInterface
+ *       (src/speciesTransport/species/src/mixture.cpp:400-528,
+ *        src/speciesTransport/reactions/cuda/include/cuChemInterface.hpp:61-310)
+ *       and the CHEMKIN reader chemkinFileRead
+ *       (src/speciesTransport/chemistryRead/src/chemkinFileRead.cpp:1349-1700).
+ *   ohc_ctx_create
+ *       chemistrySource::chemistrySource + ODEsSolver::ODEsSolver +
+ *       ImplicitEuler::ImplicitEuler controller keys
+ *       (src/speciesTransport/chemistrySource/src/chemistrySource.cpp:41-85,
+ *        src/Hurricane/mathematicalTools/ODEs/src/ODEsSolver.cpp:80-92,
+ *        src/Hurricane/mathematicalTools/ODEs/src/ImplicitEuler.cpp:68-84).
+ *   ohc_chem_solve
+ *       real chemistrySource::solve(const realArray& dt, const real dtFactor)
+ *       (chemistrySource.hpp:622, chemistrySource.cpp:1494-1556).
+ *   ohc_chem_solve_update
+ *       void chemistrySource::solveUpdate(const realArray& dt, const real dtFactor)
+ *       (chemistrySource.hpp:628, chemistrySource.cpp:1558-1609).
+ *   ohc_chem_source
+ *       chemistrySource::calculateSourceTerms / calculateSourceTermsImp
+ *       (chemistrySource.hpp:597-598, chemistrySource.cpp:1371-1450).
+ *   ohc_eval_rhs / ohc_eval_jacobian
+ *       chemistrySource::DyDt / chemistrySource::jacobian
+ *       (chemistrySource.cpp:863-907, :967-1055).
+ *   ohc_thermo_T_from_h
+ *       thermoList::THa_p / THa_rho (src/speciesTransport/thermo/include/thermoList.inl:315-340,
+ *       thermoList.cpp:504-533).
+ *   ohc_mix_props
+ *       mixture::gasProperties / transportList::muKappa / transportList::Df and
+ *       CUDATransport::calcTransportPropertiesAsync
+ *       (src/speciesTransport/transport/src/transportList.cpp:286-427,
+ *        src/speciesTransport/transport/include/transportListCUDA.hpp:121-128).
+ *
+ * Data layout.  Cell fields are structure-of-arrays: scalars are arrays of
+ * n doubles; species fields are one block with leading dimension ld
+ * (species k of cell i is at Y[k*ld + i], ld >= n), the contiguous form of the
+ * reference's PtrList<cellRealArray> yi_[isp][cell].  Units follow the
+ * reference: T [K], p [Pa], rho [kg/m^3], Y mass fractions, dt [s],
+ * R [kg/m^3/s], concentrations [kmol/m^3].
+ *
+ * Pointers are host pointers unless ohc_opts.device_pointers is set, in which
+ * case every array argument must be device memory of the context's device
+ * and the call is asynchronous on the context stream (ohc_sync waits).
+ */
+#ifndef OHC_H
+#define OHC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OHC_OK 0
+#define OHC_ERR_ARG 1      /* invalid argument                          */
+#define OHC_ERR_PARSE 2    /* mechanism / thermo / transport file error */
+#define OHC_ERR_DEVICE 3   /* HIP runtime error                         */
+#define OHC_ERR_UNSUPPORTED 4
+#define OHC_ERR_NUMERIC 5  /* integration failed (step underflow, NaN)  */
+
+#define OHC_MAXP 6         /* max distinct species on one side of a reaction */
+#define OHC_MAXNS 64       /* max species handled by the device kernels      */
+
+/* reaction reversibility (reactionTypes::types) */
+#define OHC_RXN_IRREVERSIBLE 0
+#define OHC_RXN_REVERSIBLE 1      /* kr = kf / Kc                           */
+#define OHC_RXN_REV_PARAMS 2      /* kr from explicit REV Arrhenius params  */
+
+/* rate-constant law (reactionRateTypes) */
+#define OHC_RATE_ARRHENIUS 0
+#define OHC_RATE_THIRDBODY 1      /* ArrheniusThirdBodyRR                   */
+#define OHC_RATE_FALLOFF 2        /* unimolecularFallOffRR                  */
+#define OHC_RATE_CHEMACT 3        /* chemicallyActicatedBimolecularRR       */
+
+/* fall-off broadening function (fallOffFunctions) */
+#define OHC_FO_LINDEMANN 0
+#define OHC_FO_TROE3 1
+#define OHC_FO_TROE4 2
+#define OHC_FO_SRI 3
+
+/* Flat mechanism tables (SoA by species / by reaction).
+ * All arrays are owned by the caller of ohc_mech_from_tables (copied in) or by
+ * the ohc_mech returned by ohc_mech_get_tables (valid while it lives). */
+typedef struct ohc_mech_tables {
+    int32_t ns, nr, ne;
+    /* species */
+    const double *W;        /* [ns] molecular weight kg/kmol                     */
+    const double *Tlow;     /* [ns] NASA-7 validity range and switch temperature */
+    const double *Thigh;
+    const double *Tcommon;
+    const double *nasa_lo;  /* [ns*7] raw (dimensionless) NASA-7 a0..a6, T<Tcommon */
+    const double *nasa_hi;  /* [ns*7] raw NASA-7 a0..a6, T>=Tcommon              */
+    const double *elem;     /* [ns*ne] atoms of element e in species k            */
+    const double *ekb;      /* [ns] Lennard-Jones well depth eps/k [K]           */
+    const double *sigma;    /* [ns] Lennard-Jones diameter [Angstrom]            */
+    /* reactions */
+    const int32_t *nf, *nb;       /* [nr] distinct forward / backward participants */
+    const int32_t *f_idx, *b_idx; /* [nr*OHC_MAXP] species index                  */
+    const double *f_nu, *b_nu;    /* [nr*OHC_MAXP] stoichiometric coefficient     */
+    const double *f_ord, *b_ord;  /* [nr*OHC_MAXP] concentration exponent         */
+    const int32_t *rev;           /* [nr] OHC_RXN_*                               */
+    const int32_t *kind;          /* [nr] OHC_RATE_*                              */
+    const double *arr;            /* [nr*3] A, beta, Ta (SI kmol-m^3-s-K): kf, or
+                                     k_inf for FALLOFF, k0 for CHEMACT              */
+    const double *arr_lo;         /* [nr*3] k0 for FALLOFF, k_inf for CHEMACT      */
+    const double *arr_rev;        /* [nr*3] REV parameters (rev == REV_PARAMS)     */
+    const int32_t *fo_type;       /* [nr] OHC_FO_*                                */
+    const double *fo;             /* [nr*5] Troe alpha,T***,T*,T** / SRI a,b,c,d,e */
+    const double *eff;            /* [nr*ns] third-body efficiencies              */
+} ohc_mech_tables;
+
+/* Integrator and context options; defaults are the reference defaults. */
+typedef struct ohc_opts {
+    int32_t flow_type;      /* 0 ConstantPressure (default), 1 ConstantVolume     */
+    int32_t max_steps;      /* solver/maxstep (10000)                             */
+    double atol, rtol;      /* solver/atol, rtol (1e-15, 1e-4): Newton test       */
+    double atol_inc;        /* ImplicitEuler/ATolIncrement (1e-7): step error     */
+    double rtol_inc;        /* ImplicitEuler/RTolIncrement (1e-3)                 */
+    double safe_scale;      /* solver/safescale (0.8)                             */
+    double min_scale;       /* solver/minscale (0.25)                             */
+    double max_scale;       /* solver/maxScale (2.0)                              */
+    double alpha_inc;       /* solver/alphainc (0.25)                             */
+    double T_threshold;     /* solveOptions/TemperatureThreshold (200 K)          */
+    double T_alpha;         /* solveOptions/TemperatureAlpha (0.2)                */
+    double T_beta;          /* solveOptions/TemperatureBeta (0.25)                */
+    double dt_max;          /* chemicalMaxStep (1e15)                             */
+    double T_low, T_high;   /* thermoList TLow / THigh for T(h) limits (50, 5000) */
+    int32_t precision;      /* 0: FP64 everywhere; 1: FP32 rates/Gibbs/transport,
+                               FP64 LU, T(h) Newton and sums                       */
+    int32_t device_pointers;/* 1: array arguments are device pointers             */
+    int32_t kernel;         /* 0 auto, 1 lane-per-cell specialised, 2 generic     */
+    int32_t waves_per_cu;   /* persistent-kernel occupancy target (0 = auto)      */
+} ohc_opts;
+
+/* Per-call counters (summed over cells) for roofline accounting. */
+typedef struct ohc_stats {
+    int64_t cells;          /* cells integrated (T >= threshold)                  */
+    int64_t steps;          /* accepted sub-steps                                 */
+    int64_t attempts;       /* sub-step attempts (accepted + rejected)            */
+    int64_t newton_iters;   /* Jacobian + LU + back-solve count                   */
+    int64_t rhs_evals;      /* extra RHS evaluations (first-step error estimate)  */
+    int64_t max_cell_steps; /* largest accepted sub-step count of one cell        */
+    double dt_min;          /* min over cells of the suggested sub-step           */
+} ohc_stats;
+
+typedef struct ohc_mech ohc_mech;
+typedef struct ohc_ctx ohc_ctx;
+
+int ohc_opts_default(ohc_opts *opts);
+
+/* Mechanism.  chemkin: CHEMKIN-II reaction file; thermo: NASA-7 THERMO file
+ * (may be NULL when the CHEMKIN file carries a THERMO section); transport:
+ * CHEMKIN transport file (may be NULL: transport entry points then fail). */
+int ohc_mech_load(const char *chemkin, const char *thermo, const char *transport,
+                  ohc_mech **out);
+int ohc_mech_from_tables(const ohc_mech_tables *t, ohc_mech **out);
+int ohc_mech_get_tables(const ohc_mech *m, ohc_mech_tables *out);
+int ohc_mech_species_name(const ohc_mech *m, int32_t k, const char **name);
+/* Seeded synthetic element-balanced mechanism (C/H/O/N) of ns species and nr
+ * reactions: A log-uniform, beta, Ea uniform, frac_rev reversible,
+ * frac_tb third-body, random smooth NASA-7 and transport data. */
+int ohc_mech_synthetic(int32_t ns, int32_t nr, uint64_t seed, double frac_rev,
+                       double frac_tb, ohc_mech **out);
+/* 64-bit content hash of the tables (selects specialised kernels). */
+int ohc_mech_hash(const ohc_mech *m, uint64_t *hash);
+int ohc_mech_free(ohc_mech *m);
+
+int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_ctx **out);
+int ohc_ctx_free(ohc_ctx *ctx);
+/* Use a caller stream (hipStream_t passed as void*) instead of the context's. */
+int ohc_ctx_set_stream(ohc_ctx *ctx, void *stream);
+int ohc_sync(ohc_ctx *ctx);
+const char *ohc_last_error(const ohc_ctx *ctx);
+/* Name of the integrator kernel the context selected ("lane:<mech>" or "generic"). */
+const char *ohc_ctx_kernel_name(const ohc_ctx *ctx);
+
+/* chemistrySource::solve(dt, dtFactor).  For each cell: T < T_threshold ->
+ * R = 0 and subDt = dt*dtFactor; otherwise integrate [0, dt*dtFactor] at
+ * constant p (or rho) from c = rho*Y/W, write R_k = (c_k - c0_k) W_k / (dt*dtFactor)
+ * and the suggested sub-step into subDt.  subDt is read as the starting
+ * sub-step (dtInit); a value <= 0 means dt*dtFactor (the reference's first
+ * call).  stats (host, nullable) receives counters and min subDt. */
+int ohc_chem_solve(ohc_ctx *ctx, int64_t n, const double *T, const double *p,
+                   const double *rho, const double *Y, int64_t ldY, const double *dt,
+                   double dtFactor, double *subDt, double *R, int64_t ldR,
+                   ohc_stats *stats);
+
+/* chemistrySource::solveUpdate(dt, dtFactor): Y and T updated in place. */
+int ohc_chem_solve_update(ohc_ctx *ctx, int64_t n, double *T, const double *p,
+                          const double *rho, double *Y, int64_t ldY, const double *dt,
+                          double dtFactor, double *subDt, ohc_stats *stats);
+
+/* calculateSourceTerms(false) (diagJ == NULL) and calculateSourceTermsImp
+ * (diagJ != NULL, diagonal d(omega_k)/d(c_k) incl. the temperature coupling
+ * term; withoutLastSpec selects the omegaCoupled last-species substitution). */
+int ohc_chem_source(ohc_ctx *ctx, int64_t n, const double *T, const double *p,
+                    const double *rho, const double *Y, int64_t ldY, double *R,
+                    int64_t ldR, double *diagJ, int64_t ldD, int32_t withoutLastSpec);
+
+/* RHS and analytic Jacobian at ODE states y = [c_1..c_ns, T] (ld = row stride of
+ * the (ns+1) x n block); J is (ns+1)^2 x n, entry (i,j) of cell c at
+ * J[(i*(ns+1)+j)*ldJ + c].  p (or rho0 for ConstantVolume) per cell. */
+int ohc_eval_rhs(ohc_ctx *ctx, int64_t n, const double *y, int64_t ldy,
+                 const double *p, double *f, int64_t ldf);
+int ohc_eval_jacobian(ohc_ctx *ctx, int64_t n, const double *y, int64_t ldy,
+                      const double *p, double *f, int64_t ldf, double *J, int64_t ldJ);
+
+/* thermoList::THa_p: T from absolute enthalpy h [J/kg] at pressure p and
+ * composition Y, Newton from T_inout (tol 1e-4*T0, 100 iterations, limited to
+ * [T_low, T_high]); iflag (nullable) receives 0/1/2 as thermoList::limit. */
+int ohc_thermo_T_from_h(ohc_ctx *ctx, int64_t n, const double *h, const double *p,
+                        const double *Y, int64_t ldY, double *T_inout, int32_t *iflag);
+
+/* Mixture thermo + transport at (T, p, Y): any output may be NULL.
+ * mu [kg/m/s] and kappa [W/m/K] by kinetic theory + Wilke; Dk [m^2/s]
+ * mixture-averaged (ld ldD); cp [J/kg/K]; ha [J/kg]; hk species h_a [J/kg] (ld ldD). */
+int ohc_mix_props(ohc_ctx *ctx, int64_t n, const double *T, const double *p,
+                  const double *Y, int64_t ldY, double *mu, double *kappa, double *Dk,
+                  int64_t ldD, double *cp, double *ha, double *hk);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OHC_H */

@@ -1,0 +1,279 @@
+"""Python mirror of the reference chemistry call surface over libohc.so.
+
+``ChemistrySource`` mirrors OpenHurricane's ``chemistrySource`` entry points
+(src/speciesTransport/chemistrySource/include/chemistrySource.hpp:576-662):
+``solve(dt, dtFactor)`` returns the source terms R and the minimum suggested
+sub-step, ``solve_update`` advances Y and T in place, and the thermo /
+transport calls mirror ``thermoList::THa_p`` and ``transportList::muKappa`` /
+``Df``.  Arrays are numpy arrays (host pointers) or, for a context created
+with ``device_pointers=True``, torch CUDA tensors (device pointers, calls are
+asynchronous on the context stream).  Species fields are (ns, n) blocks,
+species k of cell i at Y[k, i].
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+from ._lib import MechTables, Stats, check, lib
+
+DATA = Path(__file__).resolve().parent / "data"
+
+
+def _as_np(ptr, n, dtype=np.float64):
+    if not ptr or n == 0:
+        return np.zeros(n, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+class Mechanism:
+    """A loaded reaction mechanism (species, NASA-7 thermo, reactions, transport)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if isinstance(handle, int) else handle
+
+    # -- constructors -------------------------------------------------
+    @classmethod
+    def load(cls, chemkin, thermo=None, transport=None):
+        h = C.c_void_p()
+        enc = lambda s: None if s is None else str(s).encode()
+        check(lib().ohc_mech_load(enc(chemkin), enc(thermo), enc(transport), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def h2air(cls, transport=True):
+        """The 9-species / 19-reaction H2-air mechanism shipped with the package."""
+        d = DATA / "h2air"
+        return cls.load(d / "mech.che", d / "thermo.dat", d / "transport.dat" if transport else None)
+
+    @classmethod
+    def synthetic(cls, ns, nr, seed, frac_rev=0.2, frac_tb=0.1):
+        h = C.c_void_p()
+        check(lib().ohc_mech_synthetic(ns, nr, seed, frac_rev, frac_tb, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_tables(cls, tab: dict):
+        keep = []
+
+        def dptr(name):
+            a = tab.get(name)
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(C.c_double))
+
+        def iptr(name):
+            a = np.ascontiguousarray(tab[name], dtype=np.int32)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+        t = MechTables()
+        t.ns, t.nr, t.ne = int(tab["ns"]), int(tab["nr"]), int(tab.get("ne", 0))
+        for f in ["W", "Tlow", "Thigh", "Tcommon", "nasa_lo", "nasa_hi", "elem", "ekb", "sigma",
+                  "f_nu", "b_nu", "f_ord", "b_ord", "arr", "arr_lo", "arr_rev", "fo", "eff"]:
+            setattr(t, f, dptr(f))
+        for f in ["nf", "nb", "f_idx", "b_idx", "rev", "kind", "fo_type"]:
+            setattr(t, f, iptr(f))
+        h = C.c_void_p()
+        check(lib().ohc_mech_from_tables(C.byref(t), C.byref(h)))
+        return cls(h)
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().ohc_mech_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    # -- accessors ------------------------------------------------------
+    def raw_tables(self) -> MechTables:
+        t = MechTables()
+        check(lib().ohc_mech_get_tables(self._h, C.byref(t)))
+        return t
+
+    def tables(self) -> dict:
+        t = self.raw_tables()
+        ns, nr, ne, P = t.ns, t.nr, t.ne, _lib.OHC_MAXP
+        d = {"ns": ns, "nr": nr, "ne": ne}
+        for f, n in [("W", ns), ("Tlow", ns), ("Thigh", ns), ("Tcommon", ns), ("ekb", ns),
+                     ("sigma", ns)]:
+            d[f] = _as_np(getattr(t, f), n) if getattr(t, f) else None
+        d["nasa_lo"] = _as_np(t.nasa_lo, 7 * ns).reshape(ns, 7)
+        d["nasa_hi"] = _as_np(t.nasa_hi, 7 * ns).reshape(ns, 7)
+        d["elem"] = _as_np(t.elem, ns * ne).reshape(ns, ne)
+        for f in ["nf", "nb", "rev", "kind", "fo_type"]:
+            d[f] = _as_np(getattr(t, f), nr, np.int32)
+        for f in ["f_idx", "b_idx"]:
+            d[f] = _as_np(getattr(t, f), nr * P, np.int32).reshape(nr, P)
+        for f in ["f_nu", "b_nu", "f_ord", "b_ord"]:
+            d[f] = _as_np(getattr(t, f), nr * P).reshape(nr, P)
+        for f in ["arr", "arr_lo", "arr_rev"]:
+            d[f] = _as_np(getattr(t, f), nr * 3).reshape(nr, 3)
+        d["fo"] = _as_np(t.fo, nr * 5).reshape(nr, 5)
+        d["eff"] = _as_np(t.eff, nr * ns).reshape(nr, ns)
+        return d
+
+    @property
+    def ns(self):
+        return self.raw_tables().ns
+
+    @property
+    def nr(self):
+        return self.raw_tables().nr
+
+    @property
+    def species(self):
+        out = []
+        for k in range(self.ns):
+            s = C.c_char_p()
+            check(lib().ohc_mech_species_name(self._h, k, C.byref(s)))
+            out.append(s.value.decode())
+        return out
+
+    @property
+    def hash(self):
+        h = C.c_uint64()
+        check(lib().ohc_mech_hash(self._h, C.byref(h)))
+        return h.value
+
+
+def _ptr(a, device, dtype="float64"):
+    """Raw pointer of a numpy array (host) or torch tensor (device)."""
+    if a is None:
+        return None
+    if device:
+        import torch
+
+        if not isinstance(a, torch.Tensor) or not a.is_cuda:
+            raise TypeError("device-pointer context expects torch CUDA tensors")
+        want = torch.float64 if dtype == "float64" else torch.int32
+        if a.dtype != want or not a.is_contiguous():
+            raise TypeError(f"expected contiguous {want} tensor")
+        return C.c_void_p(a.data_ptr())
+    if not isinstance(a, np.ndarray) or a.dtype != np.dtype(dtype) or not a.flags.c_contiguous:
+        raise TypeError(f"expected C-contiguous numpy {dtype} array")
+    return C.c_void_p(a.ctypes.data)
+
+
+def _ld(a, n):
+    """Leading dimension (row stride in elements) of an (ns, n) block."""
+    if a is None:
+        return n
+    if a.ndim == 1:
+        return n
+    st = a.stride(0) if hasattr(a, "stride") and callable(a.stride) else a.strides[0] // a.itemsize
+    return int(st)
+
+
+class ChemistrySource:
+    """chemistrySource / thermoList / transportList calls on one GPU context."""
+
+    def __init__(self, mech: Mechanism, device=0, opts=None, **optkw):
+        self.mech = mech
+        self.opts = opts if opts is not None else _lib.default_opts(**optkw)
+        self.device_pointers = bool(self.opts.device_pointers)
+        self.ns = mech.ns
+        self.device = device
+        h = C.c_void_p()
+        check(lib().ohc_ctx_create(mech._h, C.byref(self.opts), device, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().ohc_ctx_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def kernel_name(self):
+        return lib().ohc_ctx_kernel_name(self._h).decode()
+
+    def set_stream(self, stream_ptr: int):
+        check(lib().ohc_ctx_set_stream(self._h, C.c_void_p(stream_ptr)), self._h)
+
+    def sync(self):
+        check(lib().ohc_sync(self._h), self._h)
+
+    def _p(self, a, dtype="float64"):
+        return _ptr(a, self.device_pointers, dtype)
+
+    # chemistrySource::solve(dt, dtFactor)
+    def solve(self, T, p, rho, Y, dt, dtFactor=1.0, subDt=None, R=None, stats=True):
+        n = T.shape[0]
+        if subDt is None:
+            subDt = self._zeros(n)
+        if R is None:
+            R = self._zeros((self.ns, n))
+        st = Stats() if stats else None
+        rc = lib().ohc_chem_solve(self._h, n, self._p(T), self._p(p), self._p(rho), self._p(Y),
+                                  _ld(Y, n), self._p(dt), float(dtFactor), self._p(subDt),
+                                  self._p(R), _ld(R, n), C.byref(st) if st else None)
+        check(rc, self._h)
+        return R, subDt, (st.as_dict() if st else None)
+
+    # chemistrySource::solveUpdate(dt, dtFactor): T and Y advanced in place
+    def solve_update(self, T, p, rho, Y, dt, dtFactor=1.0, subDt=None, stats=True):
+        n = T.shape[0]
+        if subDt is None:
+            subDt = self._zeros(n)
+        st = Stats() if stats else None
+        rc = lib().ohc_chem_solve_update(self._h, n, self._p(T), self._p(p), self._p(rho),
+                                         self._p(Y), _ld(Y, n), self._p(dt), float(dtFactor),
+                                         self._p(subDt), C.byref(st) if st else None)
+        check(rc, self._h)
+        return subDt, (st.as_dict() if st else None)
+
+    # chemistrySource::DyDt / jacobian at states y (N, n)
+    def eval_rhs(self, y, p):
+        n = y.shape[1]
+        f = self._zeros(y.shape)
+        check(lib().ohc_eval_rhs(self._h, n, self._p(y), _ld(y, n), self._p(p), self._p(f),
+                                 _ld(f, n)), self._h)
+        return f
+
+    def eval_jacobian(self, y, p):
+        N, n = y.shape
+        f = self._zeros(y.shape)
+        J = self._zeros((N * N, n))
+        check(lib().ohc_eval_jacobian(self._h, n, self._p(y), _ld(y, n), self._p(p), self._p(f),
+                                      _ld(f, n), self._p(J), _ld(J, n)), self._h)
+        return f, J
+
+    # thermoList::THa_p
+    def T_from_h(self, h, p, Y, T0, want_flag=False):
+        n = h.shape[0]
+        T = T0.copy() if not self.device_pointers else T0.clone()
+        flag = self._zeros(n, np.int32) if want_flag else None
+        check(lib().ohc_thermo_T_from_h(self._h, n, self._p(h), self._p(p), self._p(Y),
+                                        _ld(Y, n), self._p(T),
+                                        self._p(flag, "int32") if want_flag else None), self._h)
+        return (T, flag) if want_flag else T
+
+    # transportList::muKappa + Df, thermoList cp / ha
+    def mix_props(self, T, p, Y, mu=True, kappa=True, Dk=True, cp=True, ha=True, hk=False):
+        n = T.shape[0]
+        out = {}
+        for key, want, shape in [("mu", mu, n), ("kappa", kappa, n), ("Dk", Dk, (self.ns, n)),
+                                 ("cp", cp, n), ("ha", ha, n), ("hk", hk, (self.ns, n))]:
+            out[key] = self._zeros(shape) if want else None
+        check(lib().ohc_mix_props(self._h, n, self._p(T), self._p(p), self._p(Y), _ld(Y, n),
+                                  self._p(out["mu"]), self._p(out["kappa"]), self._p(out["Dk"]),
+                                  n, self._p(out["cp"]), self._p(out["ha"]), self._p(out["hk"])),
+              self._h)
+        return {k: v for k, v in out.items() if v is not None}
+
+    def _zeros(self, shape, dtype=np.float64):
+        if self.device_pointers:
+            import torch
+
+            tdt = torch.float64 if dtype == np.float64 else torch.int32
+            return torch.zeros(shape, dtype=tdt, device=f"cuda:{self.device}")
+        return np.zeros(shape, dtype=dtype)

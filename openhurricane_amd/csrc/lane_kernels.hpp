@@ -1,0 +1,583 @@
+// lane_kernels.hpp -- persistent lane-per-cell kernels specialised on a
+// compile-time mechanism M (see chem_device.hpp).
+//
+// ie_lane_kernel restates, per cell and in FP64, the reference's plain stiff
+// integration path:
+//   chemistrySource::solve(dt, dtFactor) / solveUpdate(dt, dtFactor)
+//       src/speciesTransport/chemistrySource/src/chemistrySource.cpp:1494-1609, :1300-1325
+//   ODESolver<chemistrySource>::solve
+//       src/speciesTransport/chemistryODESolvers/include/ODESolver.inl:74-90
+//   ImplicitEuler::{solve(t0,tn,dt0,y), adaptiveSolve, solve, NewtonIteration, maxIncrement}
+//       src/Hurricane/mathematicalTools/ODEs/src/ImplicitEuler.cpp:40-342
+//   ODEsSolver::maxError   src/Hurricane/mathematicalTools/ODEs/src/ODEsSolver.cpp:37-46
+//   chemistrySource::checkNewY   chemistrySource.cpp:1152-1164
+//
+// Work distribution: a grid of single-wavefront blocks sized to the device
+// stays resident; every lane owns one cell at a time and runs the integrator
+// as a state machine whose only expensive transition -- "evaluate f and J at
+// the Newton iterate, factor I - dt J, solve" -- is executed by all lanes of
+// the wavefront together.  A lane whose cell finishes takes the next cell
+// index from a global counter (one atomic per wavefront per refill), so cells
+// of very different stiffness never leave lanes idle while work remains.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "chem_device.hpp"
+
+namespace ohcd {
+
+struct LaneParams {
+    int64_t n;
+    const double *T, *p, *rho, *Y, *dt;
+    int64_t ldY;
+    double dtFactor;
+    double *subDt;
+    double *R;
+    int64_t ldR;
+    double *Tout, *Yout;  // update mode: written in place (may alias T / Y)
+    int mode;             // 0: chemistrySource::solve (R), 1: solveUpdate
+    int flow;             // 0 constant pressure, 1 constant volume
+    int max_steps;
+    double atol, rtol, atol_inc, rtol_inc, safe, minsc, maxsc, alpha;
+    double Tthr, Talpha, Tbeta, dt_max;
+    long long max_cell_iters;     // safety cap on Newton iterations per cell (termination)
+    unsigned long long *counter;  // work-queue head, zeroed before launch
+    unsigned long long *stats;    // see ohc_stats: [0..5] counters, [6] dt_min bits, [7] failures
+};
+
+enum : int { PH_NEED = 0, PH_NEWTON = 1, PH_ERR0 = 2, PH_DONE = 3 };
+
+__device__ __forceinline__ double wave_min(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ long long wave_sum(long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ long long wave_max(long long v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        long long u = __shfl_xor(v, o);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+template <class M>
+__global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
+    constexpr int NS = M::NS, N = NS + 1;
+    const int lane = threadIdx.x;
+
+    double y[N], yk[N], yl1[N], f[N], J[N][N], b[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) y[i] = yk[i] = yl1[i] = 0.0;
+    yk[NS] = y[NS] = 1000.0;
+
+    int64_t cell = -1;
+    int phase = PH_NEED;
+    double pr = 0.0, dtc = 0.0, leftTime = 0.0, subDt = 0.0;
+    double tn = 0.0, t = 0.0, dt = 0.0, dtTry0 = 0.0, dtl = 0.0, lts0 = 0.0, lts1 = 0.0;
+    double lastErr = 0.0;
+    int nstep = 0, ncount = 0, chk = 0;
+    bool last = false;
+    long long st_steps = 0, st_att = 0, st_newt = 0, st_rhs = 0, st_cells = 0, st_maxs = 0,
+              st_fail = 0, cell_steps = 0, cell_iters = 0;
+    double st_dtmin = __builtin_huge_val();
+
+    const double err_grow = pow(P.maxsc / P.safe, -1.0 / P.alpha);
+
+    // ---- state machine pieces (lane-divergent, cheap) ----
+    auto finish_cell = [&]() __attribute__((always_inline)) {
+        P.subDt[cell] = subDt;
+        if (P.mode == 0) {
+            const double rho = P.rho[cell], rdt = 1.0 / dtc;
+            sfor<0, NS>([&](auto k) {
+                const double c0 = rho * P.Y[k * P.ldY + cell] / M::W[k];
+                P.R[k * P.ldR + cell] = (y[k] - c0) * rdt * M::W[k];
+            });
+        } else {
+            // chemistrySource::ciToYi(ci, yi) (chemistrySource.cpp:1222-1240)
+            double yy[NS], rs = 0.0, ys = 0.0;
+            sfor<0, NS>([&](auto k) {
+                yy[k] = fmax(0.0, y[k]);
+                rs += yy[k] * M::W[k];
+            });
+            const double rr = 1.0 / fmax(rs, kTiny);
+            sfor<0, NS>([&](auto k) {
+                yy[k] = yy[k] * M::W[k] * rr;
+                ys += yy[k];
+            });
+            const double ri = 1.0 / ys;
+            sfor<0, NS>([&](auto k) { P.Yout[k * P.ldY + cell] = ys != 1.0 ? yy[k] * ri : yy[k]; });
+            P.Tout[cell] = y[NS];
+        }
+        st_cells++;
+        st_dtmin = fmin(st_dtmin, subDt);
+        st_maxs = cell_steps > st_maxs ? cell_steps : st_maxs;
+        phase = PH_NEED;
+    };
+    auto start_attempt = [&]() __attribute__((always_inline)) {
+        lts0 = dtl;
+        st_att++;
+#pragma unroll
+        for (int i = 0; i < N; ++i) yk[i] = y[i];
+        ncount = 0;
+        lastErr = 0.0;
+        phase = PH_NEWTON;
+    };
+    // ImplicitEuler::solve(t0, tn, dt0, y) loop head for step nstep; false when
+    // the step budget is exhausted (tn then becomes the time reached).
+    auto start_step = [&]() __attribute__((always_inline)) -> bool {
+        if (nstep >= P.max_steps) {
+            tn = t;
+            return false;
+        }
+        dtTry0 = dt;
+        if ((t + dt - tn) * (t + dt) > 0) {
+            dt = tn - t;
+            last = true;
+        } else if (t + dt == tn) {
+            last = true;
+        }
+        dtl = dt;  // adaptiveSolve entry
+        lts1 = lts0;
+        chk = 0;
+        start_attempt();
+        return true;
+    };
+    auto start_ode = [&]() __attribute__((always_inline)) {
+        tn = leftTime;
+        t = 0.0;
+        dt = 0.2 * subDt;
+        last = false;
+        nstep = 0;
+        return start_step();
+    };
+    // ImplicitEuler::maxIncrement
+    auto max_increment = [&](const double(&e)[N]) __attribute__((always_inline)) {
+        double m = 0.0;
+        bool nan_y = false;
+        sfor<0, N>([&](auto i) {
+            nan_y |= isnan(yk[i]);
+            const double tol = P.atol_inc + P.rtol_inc * fmax(fabs(y[i]), fabs(yk[i]));
+            m = fmax(m, fabs(e[i]) / tol);
+        });
+        return nan_y ? 1.5 : m;
+    };
+    // adaptiveSolve after one ImplicitEuler::solve attempt returned `error`,
+    // followed by the ImplicitEuler::solve and chemistrySource::solve loops.
+    enum { A_RETRY, A_FAIL, A_NEXT, A_ODE_END };
+    auto attempt_done = [&](double error) __attribute__((always_inline)) {
+        int action;
+        const bool ok = !((yk[NS] - y[NS]) > P.Talpha * y[NS]);
+        if (!ok) dtl *= P.Tbeta;
+        if (!ok && chk < 100) {
+            chk++;
+            action = dtl <= kVeryTiny ? A_FAIL : A_RETRY;
+        } else if (error > 1.0) {
+            dtl *= fmax(P.safe * pow(error, -P.alpha), P.minsc);
+            action = dtl <= kVeryTiny ? A_FAIL : A_RETRY;
+        } else {
+            t += dtl;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                yl1[i] = y[i];
+                y[i] = yk[i];
+            }
+            if (error > err_grow)
+                dt = fmin(fmax(P.safe * pow(error, -P.alpha), P.minsc), P.maxsc) * dtl;
+            else
+                dt = fmax(P.safe * P.maxsc * dtl, dtl);
+            st_steps++;
+            cell_steps++;
+            if ((t - tn) * tn >= 0) {
+                if (nstep > 0 && last) subDt = fmin(dtTry0, tn);
+                action = A_ODE_END;
+            } else {
+                nstep++;
+                action = A_NEXT;
+            }
+        }
+        if (action == A_RETRY) {
+            start_attempt();
+        } else if (action == A_NEXT) {
+            if (!start_step()) action = A_ODE_END;
+        }
+        if (action == A_ODE_END) {
+            // chemistrySource::solve: while (leftTime > 0) { dt = leftTime; solve(c, dt, subDt); leftTime -= dt; }
+            leftTime -= tn;
+            if (leftTime > 0.0) start_ode();
+            else finish_cell();
+        } else if (action == A_FAIL) {
+            st_fail++;
+            finish_cell();
+        }
+    };
+    // start a fresh cell (chemistrySource::solve / solveUpdate loop body)
+    auto start_cell = [&]() __attribute__((always_inline)) {
+        const double T0 = P.T[cell];
+        dtc = P.dt[cell] * P.dtFactor;
+        const double dtI = P.subDt[cell] > 0.0 ? P.subDt[cell] : dtc;
+        if (T0 < P.Tthr) {
+            P.subDt[cell] = dtc;
+            if (P.mode == 0) sfor<0, NS>([&](auto k) { P.R[k * P.ldR + cell] = 0.0; });
+            return;  // stays PH_NEED
+        }
+        const double rho = P.rho[cell];
+        pr = P.flow == 0 ? P.p[cell] : rho;
+        sfor<0, NS>([&](auto k) { y[k] = rho * P.Y[k * P.ldY + cell] / M::W[k]; });
+        y[NS] = T0;
+        subDt = fmin(dtc, fmin(dtI, P.dt_max));
+        leftTime = dtc;
+        cell_steps = 0;
+        cell_iters = 0;
+        start_ode();
+    };
+
+    for (;;) {
+        // ---- refill lanes that need a cell (one atomic per wavefront) ----
+        for (;;) {
+            const unsigned long long need = __ballot(phase == PH_NEED);
+            if (need == 0ull) break;
+            const int leader = __ffsll((long long)need) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(P.counter, (unsigned long long)__popcll(need));
+            base = __shfl(base, leader);
+            if (phase == PH_NEED) {
+                const unsigned long long below = lane ? (need & ((1ull << lane) - 1ull)) : 0ull;
+                const unsigned long long mine = base + (unsigned long long)__popcll(below);
+                if (mine >= (unsigned long long)P.n) {
+                    phase = PH_DONE;
+                } else {
+                    cell = (int64_t)mine;
+                    start_cell();
+                }
+            }
+        }
+        if (!__any(phase == PH_NEWTON || phase == PH_ERR0)) break;
+
+        // ---- the wavefront-wide heavy step ----
+        eval_fJ<M, true>(yk, pr, P.flow, f, J);
+        if (__any(phase == PH_NEWTON)) {
+            // (I - dt J) delta = -yk + y0 + dt f  (ImplicitEuler.cpp:186-198)
+            sfor<0, N>([&](auto i) {
+                sfor<0, N>([&](auto j) { J[i][j] = -dtl * J[i][j]; });
+                J[i][i] += 1.0;
+                b[i] = -yk[i] + y[i] + dtl * f[i];
+            });
+            lu_solve<N>(J, b);
+        }
+
+        // ---- lane-local transitions ----
+        bool have = false;
+        double aerr = 0.0;
+        if (phase == PH_NEWTON) {
+            // NewtonIteration body after the solve (ImplicitEuler.cpp:200-249)
+            st_newt++;
+            bool stop = false;
+            sfor<0, N>([&](auto i) {
+                if (!stop) {
+                    if (bad(b[i])) stop = true;
+                    else yk[i] += b[i];
+                }
+            });
+            if (stop) {
+                have = true;
+                aerr = 1000.0;
+            } else {
+                double maxErr = 0.0;
+                sfor<0, N>([&](auto i) {
+                    const double tol = P.atol + P.rtol * fabs(yk[i]);
+                    maxErr = fmax(maxErr, fabs(b[i]) / tol);
+                });
+                const bool diverged =
+                    ncount > 0 && (maxErr / fmax(lastErr, kTiny)) > 2 && ncount > 5;
+                if (diverged) {
+                    have = true;
+                    aerr = 1000.0;
+                } else {
+                    lastErr = maxErr;
+                    ncount++;
+                    if (isnan(maxErr)) {
+                        have = true;
+                        aerr = 1000.0;
+                    } else if (ncount > 1000) {
+                        have = true;
+                        aerr = maxErr;
+                    } else if (maxErr > 1.0) {
+                        // iterate again at the new yk
+                    } else if (nstep == 0) {
+                        phase = PH_ERR0;
+                    } else {
+                        // second-difference LTE (ImplicitEuler.cpp:122-139)
+                        const double dtn = lts0, dtn1 = lts1;
+                        const double f1 = 2.0 / (dtn * (dtn + dtn1));
+                        const double f2 = 2.0 / (dtn1 * (dtn + dtn1));
+                        const double LTEF = dtn * dtn * 0.5;
+                        double e[N];
+                        sfor<0, N>([&](auto i) {
+                            e[i] = (f1 * (yk[i] - y[i]) - f2 * (y[i] - yl1[i])) * LTEF;
+                        });
+                        have = true;
+                        aerr = max_increment(e);
+                    }
+                }
+            }
+        } else if (phase == PH_ERR0) {
+            // first step: e = y - y0 - dt f(y) (ImplicitEuler.cpp:111-120)
+            st_rhs++;
+            double e[N];
+            sfor<0, N>([&](auto i) { e[i] = yk[i] - y[i] - dtl * f[i]; });
+            have = true;
+            aerr = max_increment(e);
+        }
+        if (have) {
+            attempt_done(aerr);
+        } else if (phase == PH_NEWTON && ++cell_iters > P.max_cell_iters) {
+            // no legitimate cell gets here; the cap guarantees every wavefront drains
+            st_fail++;
+            finish_cell();
+        }
+    }
+
+    // ---- counters: one atomic per wavefront ----
+    const long long s0 = wave_sum(st_cells), s1 = wave_sum(st_steps), s2 = wave_sum(st_att),
+                    s3 = wave_sum(st_newt), s4 = wave_sum(st_rhs), s5 = wave_max(st_maxs),
+                    s7 = wave_sum(st_fail);
+    const double dmin = wave_min(st_dtmin);
+    if (lane == 0) {
+        atomicAdd(&P.stats[0], (unsigned long long)s0);
+        atomicAdd(&P.stats[1], (unsigned long long)s1);
+        atomicAdd(&P.stats[2], (unsigned long long)s2);
+        atomicAdd(&P.stats[3], (unsigned long long)s3);
+        atomicAdd(&P.stats[4], (unsigned long long)s4);
+        atomicMax(&P.stats[5], (unsigned long long)s5);
+        if (dmin < __builtin_huge_val())
+            atomicMin(&P.stats[6], (unsigned long long)__double_as_longlong(dmin));
+        atomicAdd(&P.stats[7], (unsigned long long)s7);
+    }
+}
+
+// ---- RHS / Jacobian evaluation at given states (chemistrySource::DyDt / jacobian) ----
+struct EvalParams {
+    int64_t n;
+    const double *y;
+    int64_t ldy;
+    const double *pr;
+    double *f;
+    int64_t ldf;
+    double *J;
+    int64_t ldJ;
+    int flow;
+};
+
+template <class M>
+__global__ __launch_bounds__(64) void eval_lane_kernel(const EvalParams P) {
+    constexpr int NS = M::NS, N = NS + 1;
+    const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= P.n) return;
+    double y[N], f[N], J[N][N];
+    sfor<0, N>([&](auto i) { y[i] = P.y[i * P.ldy + cell]; });
+    eval_fJ<M, true>(y, P.pr[cell], P.flow, f, J);
+    sfor<0, N>([&](auto i) { P.f[i * P.ldf + cell] = f[i]; });
+    if (P.J)
+        sfor<0, N>([&](auto i) {
+            sfor<0, N>([&](auto j) { P.J[(i * N + j) * P.ldJ + cell] = J[i][j]; });
+        });
+}
+
+// ---- mixture thermo + transport (transportList::muKappa, Df; thermoList cp, ha) ----
+struct PropParams {
+    int64_t n;
+    const double *T, *p, *Y;
+    int64_t ldY;
+    double *mu, *kappa, *Dk;
+    int64_t ldD;
+    double *cp, *ha, *hk;
+};
+
+template <class M>
+__global__ __launch_bounds__(256) void props_lane_kernel(const PropParams P) {
+    constexpr int NS = M::NS;
+    const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= P.n) return;
+    const double T = P.T[cell], p = P.p[cell];
+    double yi[NS], xi[NS], cpi[NS], mui[NS], kap[NS];
+    sfor<0, NS>([&](auto k) { yi[k] = P.Y[k * P.ldY + cell]; });
+    const TFac tf = tfac(T);
+    // species cp0 and ha0 [J/kg] (JANAF.cpp:155-189)
+    double cpm = 0.0, ham = 0.0;
+    sfor<0, NS>([&](auto k) {
+        Thermo<M> th;
+        species_thermo<M, k>(T, tf.lnT, tf.invTm, th);
+        cpi[k] = th.Cp[k] / M::W[k];
+        const double h = th.H[k] / M::W[k];
+        cpm += cpi[k] * yi[k];
+        ham += h * yi[k];
+        if (P.hk) P.hk[k * P.ldD + cell] = h;
+    });
+    if (P.cp) P.cp[cell] = cpm;
+    if (P.ha) P.ha[cell] = ham;
+    // Yi2Xi (speciesList.cpp:208-222)
+    if constexpr (NS == 1) {
+        xi[0] = 1.0;
+    } else {
+        double wm = 0.0, xt = 0.0;
+        sfor<0, NS>([&](auto k) { wm += yi[k] * (1.0 / M::W[k]); });
+        wm = 1.0 / fmax(wm, kVeryTiny);
+        sfor<0, NS>([&](auto k) {
+            xi[k] = fmax(0.0, fmin(1.0, yi[k] * wm / M::W[k]));
+            xt += xi[k];
+        });
+        if (xt != 1.0) {
+            const double r = 1.0 / xt;
+            sfor<0, NS>([&](auto k) { xi[k] *= r; });
+        }
+    }
+    // kineticTheory::mu / kappa (kineticTheory.cpp:37-57, kineticTheory.inl:32-34)
+    sfor<0, NS>([&](auto k) {
+        const double Ts = T * (1.0 / M::EKB[k]);
+        const double om = 1.147 * exp(-0.145 * log(Ts)) + 1.0 / ((Ts + 0.5) * (Ts + 0.5));
+        mui[k] = 2.6693e-6 * sqrt(M::W[k] * T) / (M::SIGMA[k] * M::SIGMA[k] * om);
+        kap[k] = mui[k] * M::Ri[k] * (cpi[k] / M::Ri[k] + 1.25);
+    });
+    if (P.mu || P.kappa) {
+        double mum = 0.0, km = 0.0;
+        if constexpr (NS == 1) {
+            mum = mui[0];
+            km = kap[0];
+        } else {
+            // Wilke (transportList.cpp:309-333)
+            sfor<0, NS>([&](auto i) {
+                double phi = 0.0;
+                sfor<0, NS>([&](auto j) {
+                    constexpr double Wi = M::W[i], Wj = M::W[j];
+                    const double phiP = (1.0 / sqrt(8.0)) / sqrt(1.0 + Wi / Wj);
+                    const double tt = 1.0 + sqrt(mui[i] / mui[j] * sqrt(Wj / Wi));
+                    phi += phiP * xi[j] * (tt * tt);
+                });
+                const double rphi = 1.0 / phi;
+                mum += mui[i] * xi[i] * rphi;
+                km += kap[i] * xi[i] * rphi;
+            });
+        }
+        if (P.mu) P.mu[cell] = mum;
+        if (P.kappa) P.kappa[cell] = km;
+    }
+    if (P.Dk) {
+        // kineticTheory::Di (kineticTheory.cpp:118-168)
+        const double Tv1p5 = T * sqrt(T), rp = 1.0 / p;
+        sfor<0, NS>([&](auto i) {
+            double sum = 0.0;
+            sfor<0, NS>([&](auto j) {
+                if constexpr (i != j) {
+                    constexpr double Wi = M::W[i], Wj = M::W[j];
+                    const double sij = 0.5 * (M::SIGMA[i] + M::SIGMA[j]);
+                    const double cst = 1.858e-7 * sqrt(1.0 / Wi + 1.0 / Wj) / (sij * sij) * kPatm;
+                    const double Ts = T * (1.0 / sqrt(M::EKB[i] * M::EKB[j]));
+                    const double Od = exp(-0.145 * log(Ts)) + 1.0 / ((Ts + 0.5) * (Ts + 0.5));
+                    const double Dij = Tv1p5 * cst * rp / Od;
+                    sum += xi[j] / Dij;
+                }
+            });
+            double Dim = xi[i] < 1.0 - kVeryTiny ? (1.0 - xi[i]) / sum : 0.0;
+            if (isinf(Dim)) Dim = 0.0;
+            P.Dk[i * P.ldD + cell] = Dim;
+        });
+    }
+}
+
+// ---- thermoList::THa_p Newton inversion (thermoList.cpp:504-533) ----
+struct ThParams {
+    int64_t n;
+    const double *h, *Y;
+    int64_t ldY;
+    double *T;
+    int *iflag;
+    double Tlow, Thigh;
+};
+
+template <class M>
+__global__ __launch_bounds__(256) void tfromh_lane_kernel(const ThParams P) {
+    constexpr int NS = M::NS;
+    const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= P.n) return;
+    double yi[NS];
+    sfor<0, NS>([&](auto k) { yi[k] = P.Y[k * P.ldY + cell]; });
+    const double hstar = P.h[cell];
+    const double T0 = P.T[cell], Ttol = T0 * 1.0e-4;
+    double Test = T0, Tnew = T0;
+    int iter = 0, flag = 1;
+    do {
+        Test = Tnew;
+        const TFac tf = tfac(Test);
+        double hm = 0.0, cpm = 0.0;
+        sfor<0, NS>([&](auto k) {
+            Thermo<M> th;
+            species_thermo<M, k>(Test, tf.lnT, tf.invTm, th);
+            hm += th.H[k] / M::W[k] * yi[k];
+            cpm += th.Cp[k] / M::W[k] * yi[k];
+        });
+        double Tn = Test - (hm - hstar) / cpm;
+        if (Tn < P.Tlow) {
+            flag = 0;
+            Tn = P.Tlow;
+        } else if (Tn > P.Thigh) {
+            flag = 2;
+            Tn = P.Thigh;
+        } else {
+            flag = 1;
+        }
+        Tnew = Tn;
+        if (iter++ > 100) break;
+    } while (fabs(Tnew - Test) > Ttol);
+    P.T[cell] = Tnew;
+    if (P.iflag) P.iflag[cell] = flag;
+}
+
+// Host-side entry of one compiled mechanism.
+struct LaneEntry {
+    const char *name;
+    unsigned long long hash, thash;
+    int ns;
+    hipError_t (*integrate)(const LaneParams &, int grid, hipStream_t);
+    hipError_t (*eval)(const EvalParams &, hipStream_t);
+    hipError_t (*props)(const PropParams &, hipStream_t);
+    hipError_t (*tfromh)(const ThParams &, hipStream_t);
+    const void *integrate_fn;
+};
+
+template <class M>
+struct LaneLaunch {
+    static hipError_t integrate(const LaneParams &p, int grid, hipStream_t s) {
+        hipLaunchKernelGGL(ie_lane_kernel<M>, dim3(grid), dim3(64), 0, s, p);
+        return hipGetLastError();
+    }
+    static hipError_t eval(const EvalParams &p, hipStream_t s) {
+        const int g = (int)((p.n + 63) / 64);
+        hipLaunchKernelGGL(eval_lane_kernel<M>, dim3(g), dim3(64), 0, s, p);
+        return hipGetLastError();
+    }
+    static hipError_t props(const PropParams &p, hipStream_t s) {
+        const int g = (int)((p.n + 255) / 256);
+        hipLaunchKernelGGL(props_lane_kernel<M>, dim3(g), dim3(256), 0, s, p);
+        return hipGetLastError();
+    }
+    static hipError_t tfromh(const ThParams &p, hipStream_t s) {
+        const int g = (int)((p.n + 255) / 256);
+        hipLaunchKernelGGL(tfromh_lane_kernel<M>, dim3(g), dim3(256), 0, s, p);
+        return hipGetLastError();
+    }
+    static LaneEntry entry(const char *name) {
+        return LaneEntry{name,      M::HASH,      M::THASH, M::NS, &integrate, &eval,
+                         &props,    &tfromh,      (const void *)&ie_lane_kernel<M>};
+    }
+};
+
+}  // namespace ohcd
+
+// Registry of compiled mechanisms (defined in the generated registry TU).
+extern "C" const ohcd::LaneEntry *ohc_lane_entries(int *count);

@@ -1,0 +1,548 @@
+// ohc_api.cpp -- C-ABI of libohc.so (declared in include/ohc.h).
+//
+// Host side of the drop-in boundary: mechanism objects, per-device contexts
+// (one HIP stream each, no globals shared between contexts), argument checks,
+// host<->device staging for host-pointer calls and kernel dispatch.  Every
+// entry point returns a status; nothing aborts.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ohc.h"
+#include "lane_kernels.hpp"
+#include "mech.hpp"
+
+struct ohc_mech {
+    ohc::Mechanism m;
+};
+
+struct ohc_ctx {
+    ohc::Mechanism mech;
+    ohc_opts opts;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    const ohcd::LaneEntry *lane = nullptr;  // specialised kernels for this mechanism
+    bool lane_transport = false;
+    int grid = 0;  // persistent integrator grid (single-wavefront blocks)
+    unsigned long long *d_ctl = nullptr;  // [0] queue head, [1..8] stats
+    std::vector<void *> stage;            // host-pointer staging buffers
+    std::vector<size_t> stage_sz;
+    std::string err, kname;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(ohc_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(ctx, OHC_ERR_DEVICE,                                                \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+__global__ void reset_ctl(unsigned long long *ctl) {
+    const int i = threadIdx.x;
+    if (i < 9) ctl[i] = (i == 7) ? 0x7ff0000000000000ull : 0ull;  // stats[6] = +inf bits
+}
+
+bool read_file(const char *path, std::string &out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    std::ostringstream s;
+    s << f.rdbuf();
+    out = s.str();
+    return true;
+}
+
+// Device staging for host pointers: slot i holds at least bytes.
+int stage_buf(ohc_ctx *c, size_t slot, size_t bytes, void **out) {
+    if (c->stage.size() <= slot) {
+        c->stage.resize(slot + 1, nullptr);
+        c->stage_sz.resize(slot + 1, 0);
+    }
+    if (c->stage_sz[slot] < bytes) {
+        if (c->stage[slot]) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipFree(c->stage[slot]));
+        }
+        c->stage[slot] = nullptr;
+        c->stage_sz[slot] = 0;
+        HIPCHK(c, hipMalloc(&c->stage[slot], bytes));
+        c->stage_sz[slot] = bytes;
+    }
+    *out = c->stage[slot];
+    return OHC_OK;
+}
+
+// Bring one array to the device (host mode) or pass it through (device mode).
+struct Arr {
+    const void *host = nullptr;
+    void *dev = nullptr;
+    size_t bytes = 0;
+    bool out = false;
+};
+
+class Staging {
+   public:
+    explicit Staging(ohc_ctx *c) : c_(c) {}
+    // in: copy host->device; out: copy device->host at finish()
+    int map(const void *p, size_t bytes, bool in, bool out, void **dev) {
+        if (!p) {
+            *dev = nullptr;
+            return OHC_OK;
+        }
+        if (c_->opts.device_pointers) {
+            *dev = const_cast<void *>(p);
+            return OHC_OK;
+        }
+        void *d = nullptr;
+        int rc = stage_buf(c_, slot_++, bytes, &d);
+        if (rc) return rc;
+        if (in) HIPCHK(c_, hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, c_->stream));
+        if (out) arrs_.push_back(Arr{p, d, bytes, true});
+        *dev = d;
+        return OHC_OK;
+    }
+    int finish() {
+        if (c_->opts.device_pointers) return OHC_OK;
+        for (auto &a : arrs_)
+            HIPCHK(c_, hipMemcpyAsync(const_cast<void *>(a.host), a.dev, a.bytes,
+                                      hipMemcpyDeviceToHost, c_->stream));
+        HIPCHK(c_, hipStreamSynchronize(c_->stream));
+        return OHC_OK;
+    }
+
+   private:
+    ohc_ctx *c_;
+    size_t slot_ = 0;
+    std::vector<Arr> arrs_;
+};
+
+int check_ctx(ohc_ctx *c) {
+    if (!c) return fail(nullptr, OHC_ERR_ARG, "null context");
+    HIPCHK(c, hipSetDevice(c->device));
+    return OHC_OK;
+}
+
+int read_stats(ohc_ctx *c, ohc_stats *st) {
+    unsigned long long h[9];
+    HIPCHK(c, hipMemcpyAsync(h, c->d_ctl, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (st) {
+        st->cells = (int64_t)h[1];
+        st->steps = (int64_t)h[2];
+        st->attempts = (int64_t)h[3];
+        st->newton_iters = (int64_t)h[4];
+        st->rhs_evals = (int64_t)h[5];
+        st->max_cell_steps = (int64_t)h[6];
+        double d;
+        std::memcpy(&d, &h[7], sizeof(d));
+        st->dt_min = h[1] ? d : 1.0e15;
+    }
+    if (h[8]) {
+        return fail(c, OHC_ERR_NUMERIC,
+                    std::to_string(h[8]) +
+                        " cell(s) failed: step size underflow (dt <= 1e-300) or iteration cap");
+    }
+    return OHC_OK;
+}
+
+int chem_integrate(ohc_ctx *c, int mode, int64_t n, const double *T, double *Tio,
+                   const double *p, const double *rho, const double *Y, double *Yio,
+                   int64_t ldY, const double *dt, double dtFactor, double *subDt, double *R,
+                   int64_t ldR, ohc_stats *stats) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    const int ns = c->mech.ns;
+    if (n < 0) return fail(c, OHC_ERR_ARG, "n < 0");
+    if (n == 0) {
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->dt_min = 1.0e15;
+        }
+        return OHC_OK;
+    }
+    if (ldY < n || (mode == 0 && ldR < n))
+        return fail(c, OHC_ERR_ARG, "leading dimension smaller than n");
+    if (!(mode == 0 ? T : Tio) || !p || !rho || !(mode == 0 ? Y : Yio) || !dt || !subDt ||
+        (mode == 0 && !R))
+        return fail(c, OHC_ERR_ARG, "null array argument");
+    if (!(dtFactor > 0)) return fail(c, OHC_ERR_ARG, "dtFactor must be > 0");
+    if (!c->lane)
+        return fail(c, OHC_ERR_UNSUPPORTED,
+                    "no device integrator compiled for this mechanism (hash mismatch); "
+                    "specialised kernels exist for: " + c->kname);
+    const size_t sv = sizeof(double) * (size_t)n;
+    const size_t sY = sizeof(double) * (size_t)(ldY * (ns - 1) + n);
+    Staging st(c);
+    void *dT, *dp, *drho, *dY, *ddt, *dsub, *dR = nullptr;
+    if ((rc = st.map(mode == 0 ? (const void *)T : Tio, sv, true, mode == 1, &dT))) return rc;
+    if ((rc = st.map(p, sv, true, false, &dp))) return rc;
+    if ((rc = st.map(rho, sv, true, false, &drho))) return rc;
+    if ((rc = st.map(mode == 0 ? (const void *)Y : Yio, sY, true, mode == 1, &dY))) return rc;
+    if ((rc = st.map(dt, sv, true, false, &ddt))) return rc;
+    if ((rc = st.map(subDt, sv, true, true, &dsub))) return rc;
+    if (mode == 0) {
+        const size_t sR = sizeof(double) * (size_t)(ldR * (ns - 1) + n);
+        if ((rc = st.map(R, sR, false, true, &dR))) return rc;
+    }
+    ohcd::LaneParams P;
+    std::memset(&P, 0, sizeof(P));
+    const ohc_opts &o = c->opts;
+    P.n = n;
+    P.T = (const double *)dT;
+    P.p = (const double *)dp;
+    P.rho = (const double *)drho;
+    P.Y = (const double *)dY;
+    P.dt = (const double *)ddt;
+    P.ldY = ldY;
+    P.dtFactor = dtFactor;
+    P.subDt = (double *)dsub;
+    P.R = (double *)dR;
+    P.ldR = ldR;
+    P.Tout = (double *)dT;
+    P.Yout = (double *)dY;
+    P.mode = mode;
+    P.flow = o.flow_type;
+    P.max_steps = o.max_steps;
+    P.atol = o.atol;
+    P.rtol = o.rtol;
+    P.atol_inc = o.atol_inc;
+    P.rtol_inc = o.rtol_inc;
+    P.safe = o.safe_scale;
+    P.minsc = o.min_scale;
+    P.maxsc = o.max_scale;
+    P.alpha = o.alpha_inc;
+    P.Tthr = o.T_threshold;
+    P.Talpha = o.T_alpha;
+    P.Tbeta = o.T_beta;
+    P.dt_max = o.dt_max;
+    P.max_cell_iters = 50000000;
+    P.counter = c->d_ctl;
+    P.stats = c->d_ctl + 1;
+    hipLaunchKernelGGL(reset_ctl, dim3(1), dim3(64), 0, c->stream, c->d_ctl);
+    HIPCHK(c, hipGetLastError());
+    const int grid = (int)std::min<int64_t>(c->grid, (n + 63) / 64);
+    HIPCHK(c, c->lane->integrate(P, grid, c->stream));
+    if ((rc = st.finish())) return rc;
+    if (stats || !o.device_pointers) return read_stats(c, stats);
+    return OHC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ohc_opts_default(ohc_opts *o) {
+    if (!o) return fail(nullptr, OHC_ERR_ARG, "null opts");
+    std::memset(o, 0, sizeof(*o));
+    o->flow_type = 0;
+    o->max_steps = 10000;
+    o->atol = 1.0e-15;
+    o->rtol = 1.0e-4;
+    o->atol_inc = 1.0e-7;
+    o->rtol_inc = 1.0e-3;
+    o->safe_scale = 0.8;
+    o->min_scale = 0.25;
+    o->max_scale = 2.0;
+    o->alpha_inc = 0.25;
+    o->T_threshold = 200.0;
+    o->T_alpha = 0.2;
+    o->T_beta = 0.25;
+    o->dt_max = 1.0e15;
+    o->T_low = 50.0;
+    o->T_high = 5000.0;
+    return OHC_OK;
+}
+
+int ohc_mech_load(const char *chemkin, const char *thermo, const char *transport,
+                  ohc_mech **out) {
+    if (!chemkin || !out) return fail(nullptr, OHC_ERR_ARG, "null argument");
+    *out = nullptr;
+    std::string ck, th, tr;
+    if (!read_file(chemkin, ck)) return fail(nullptr, OHC_ERR_ARG, std::string("cannot read ") + chemkin);
+    if (thermo && !read_file(thermo, th)) return fail(nullptr, OHC_ERR_ARG, std::string("cannot read ") + thermo);
+    if (transport && !read_file(transport, tr))
+        return fail(nullptr, OHC_ERR_ARG, std::string("cannot read ") + transport);
+    auto m = std::make_unique<ohc_mech>();
+    try {
+        ohc::parse_chemkin(ck, thermo ? &th : nullptr, transport ? &tr : nullptr, m->m);
+    } catch (const std::exception &e) {
+        return fail(nullptr, OHC_ERR_PARSE, e.what());
+    }
+    *out = m.release();
+    return OHC_OK;
+}
+
+int ohc_mech_from_tables(const ohc_mech_tables *t, ohc_mech **out) {
+    if (!t || !out) return fail(nullptr, OHC_ERR_ARG, "null argument");
+    *out = nullptr;
+    auto m = std::make_unique<ohc_mech>();
+    try {
+        ohc::load_from_tables(*t, m->m);
+    } catch (const std::exception &e) {
+        return fail(nullptr, OHC_ERR_ARG, e.what());
+    }
+    *out = m.release();
+    return OHC_OK;
+}
+
+int ohc_mech_get_tables(const ohc_mech *m, ohc_mech_tables *out) {
+    if (!m || !out) return fail(nullptr, OHC_ERR_ARG, "null argument");
+    *out = m->m.tables();
+    if (!m->m.has_transport) {
+        out->ekb = nullptr;
+        out->sigma = nullptr;
+    }
+    return OHC_OK;
+}
+
+int ohc_mech_species_name(const ohc_mech *m, int32_t k, const char **name) {
+    if (!m || !name || k < 0 || k >= m->m.ns) return fail(nullptr, OHC_ERR_ARG, "bad species index");
+    *name = m->m.species[k].c_str();
+    return OHC_OK;
+}
+
+int ohc_mech_synthetic(int32_t ns, int32_t nr, uint64_t seed, double frac_rev, double frac_tb,
+                       ohc_mech **out) {
+    if (!out) return fail(nullptr, OHC_ERR_ARG, "null argument");
+    *out = nullptr;
+    auto m = std::make_unique<ohc_mech>();
+    try {
+        ohc::synthetic_mechanism(ns, nr, seed, frac_rev, frac_tb, m->m);
+    } catch (const std::exception &e) {
+        return fail(nullptr, OHC_ERR_ARG, e.what());
+    }
+    *out = m.release();
+    return OHC_OK;
+}
+
+int ohc_mech_hash(const ohc_mech *m, uint64_t *hash) {
+    if (!m || !hash) return fail(nullptr, OHC_ERR_ARG, "null argument");
+    *hash = m->m.hash();
+    return OHC_OK;
+}
+
+int ohc_mech_free(ohc_mech *m) {
+    delete m;
+    return OHC_OK;
+}
+
+int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_ctx **out) {
+    if (!m || !out) return fail(nullptr, OHC_ERR_ARG, "null argument");
+    *out = nullptr;
+    auto c = std::make_unique<ohc_ctx>();
+    c->mech = m->m;
+    if (opts) c->opts = *opts;
+    else ohc_opts_default(&c->opts);
+    const ohc_opts &o = c->opts;
+    if (o.flow_type != 0 && o.flow_type != 1) return fail(nullptr, OHC_ERR_ARG, "flow_type must be 0 or 1");
+    if (o.max_steps < 1) return fail(nullptr, OHC_ERR_ARG, "max_steps must be >= 1");
+    if (!(o.rtol > 0) || !(o.atol >= 0) || !(o.rtol_inc > 0) || !(o.atol_inc >= 0))
+        return fail(nullptr, OHC_ERR_ARG, "tolerances must be positive");
+    if (!(o.safe_scale > 0) || !(o.min_scale > 0) || !(o.max_scale > 0) || !(o.alpha_inc > 0) ||
+        !(o.T_beta > 0) || !(o.T_beta < 1))
+        return fail(nullptr, OHC_ERR_ARG, "step-control factors out of range");
+    if (o.precision != 0) return fail(nullptr, OHC_ERR_UNSUPPORTED, "precision mode not available");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(nullptr, OHC_ERR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(nullptr, OHC_ERR_ARG, "device index out of range");
+    c->device = device;
+    ohc_ctx *cp = c.get();
+    HIPCHK(cp, hipSetDevice(device));
+    HIPCHK(cp, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->own_stream = true;
+    HIPCHK(cp, hipMalloc((void **)&c->d_ctl, 16 * sizeof(unsigned long long)));
+    // select the specialised kernel set
+    int cnt = 0;
+    const ohcd::LaneEntry *ent = ohc_lane_entries(&cnt);
+    const unsigned long long h = c->mech.hash(),
+                             th = c->mech.has_transport ? c->mech.hash_transport() : 0ull;
+    std::string names;
+    for (int i = 0; i < cnt; ++i) {
+        names += (i ? ", " : "") + std::string(ent[i].name);
+        if (ent[i].hash == h && (o.kernel == 0 || o.kernel == 1)) {
+            c->lane = &ent[i];
+            c->lane_transport = th != 0 && ent[i].thash == th;
+        }
+    }
+    if (c->lane) {
+        hipDeviceProp_t prop;
+        HIPCHK(cp, hipGetDeviceProperties(&prop, device));
+        int per_cu = 0;
+        HIPCHK(cp, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, c->lane->integrate_fn, 64, 0));
+        if (o.waves_per_cu > 0) per_cu = std::min(per_cu, o.waves_per_cu);
+        if (per_cu < 1) per_cu = 1;
+        c->grid = prop.multiProcessorCount * per_cu;
+        c->kname = std::string("lane:") + c->lane->name;
+    } else {
+        c->kname = names.empty() ? "none" : names;
+    }
+    *out = c.release();
+    return OHC_OK;
+}
+
+int ohc_ctx_free(ohc_ctx *c) {
+    if (!c) return OHC_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->stage)
+        if (p) (void)hipFree(p);
+    if (c->d_ctl) (void)hipFree(c->d_ctl);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return OHC_OK;
+}
+
+int ohc_ctx_set_stream(ohc_ctx *c, void *stream) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->own_stream) HIPCHK(c, hipStreamDestroy(c->stream));
+    c->stream = (hipStream_t)stream;
+    c->own_stream = false;
+    return OHC_OK;
+}
+
+int ohc_sync(ohc_ctx *c) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OHC_OK;
+}
+
+const char *ohc_last_error(const ohc_ctx *c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+const char *ohc_ctx_kernel_name(const ohc_ctx *c) { return c ? c->kname.c_str() : ""; }
+
+int ohc_chem_solve(ohc_ctx *c, int64_t n, const double *T, const double *p, const double *rho,
+                   const double *Y, int64_t ldY, const double *dt, double dtFactor,
+                   double *subDt, double *R, int64_t ldR, ohc_stats *stats) {
+    return chem_integrate(c, 0, n, T, nullptr, p, rho, Y, nullptr, ldY, dt, dtFactor, subDt, R,
+                          ldR, stats);
+}
+
+int ohc_chem_solve_update(ohc_ctx *c, int64_t n, double *T, const double *p, const double *rho,
+                          double *Y, int64_t ldY, const double *dt, double dtFactor,
+                          double *subDt, ohc_stats *stats) {
+    return chem_integrate(c, 1, n, nullptr, T, p, rho, nullptr, Y, ldY, dt, dtFactor, subDt,
+                          nullptr, n, stats);
+}
+
+int ohc_chem_source(ohc_ctx *c, int64_t n, const double *T, const double *p, const double *rho,
+                    const double *Y, int64_t ldY, double *R, int64_t ldR, double *diagJ,
+                    int64_t ldD, int32_t withoutLastSpec) {
+    (void)n, (void)T, (void)p, (void)rho, (void)Y, (void)ldY, (void)R, (void)ldR, (void)diagJ,
+        (void)ldD, (void)withoutLastSpec;
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    return fail(c, OHC_ERR_UNSUPPORTED, "ohc_chem_source is not available in this build");
+}
+
+static int eval_common(ohc_ctx *c, int64_t n, const double *y, int64_t ldy, const double *p,
+                       double *f, int64_t ldf, double *J, int64_t ldJ) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    const int N = c->mech.ns + 1;
+    if (n < 0 || ldy < n || ldf < n || (J && ldJ < n)) return fail(c, OHC_ERR_ARG, "bad sizes");
+    if (!y || !p || !f) return fail(c, OHC_ERR_ARG, "null array argument");
+    if (n == 0) return OHC_OK;
+    if (!c->lane) return fail(c, OHC_ERR_UNSUPPORTED, "no device kernels for this mechanism");
+    Staging st(c);
+    void *dy, *dp, *df, *dJ = nullptr;
+    if ((rc = st.map(y, sizeof(double) * (size_t)(ldy * (N - 1) + n), true, false, &dy))) return rc;
+    if ((rc = st.map(p, sizeof(double) * (size_t)n, true, false, &dp))) return rc;
+    if ((rc = st.map(f, sizeof(double) * (size_t)(ldf * (N - 1) + n), false, true, &df))) return rc;
+    if (J && (rc = st.map(J, sizeof(double) * (size_t)(ldJ * (N * N - 1) + n), false, true, &dJ)))
+        return rc;
+    ohcd::EvalParams P{n, (const double *)dy, ldy, (const double *)dp, (double *)df, ldf,
+                       (double *)dJ, ldJ, c->opts.flow_type};
+    HIPCHK(c, c->lane->eval(P, c->stream));
+    return st.finish();
+}
+
+int ohc_eval_rhs(ohc_ctx *c, int64_t n, const double *y, int64_t ldy, const double *p,
+                 double *f, int64_t ldf) {
+    return eval_common(c, n, y, ldy, p, f, ldf, nullptr, n);
+}
+
+int ohc_eval_jacobian(ohc_ctx *c, int64_t n, const double *y, int64_t ldy, const double *p,
+                      double *f, int64_t ldf, double *J, int64_t ldJ) {
+    if (!J) return fail(c, OHC_ERR_ARG, "null J");
+    return eval_common(c, n, y, ldy, p, f, ldf, J, ldJ);
+}
+
+int ohc_thermo_T_from_h(ohc_ctx *c, int64_t n, const double *h, const double *p,
+                        const double *Y, int64_t ldY, double *T, int32_t *iflag) {
+    (void)p;
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    const int ns = c->mech.ns;
+    if (n < 0 || ldY < n) return fail(c, OHC_ERR_ARG, "bad sizes");
+    if (!h || !Y || !T) return fail(c, OHC_ERR_ARG, "null array argument");
+    if (n == 0) return OHC_OK;
+    if (!c->lane) return fail(c, OHC_ERR_UNSUPPORTED, "no device kernels for this mechanism");
+    Staging st(c);
+    void *dh, *dY, *dT, *dI = nullptr;
+    const size_t sv = sizeof(double) * (size_t)n;
+    if ((rc = st.map(h, sv, true, false, &dh))) return rc;
+    if ((rc = st.map(Y, sizeof(double) * (size_t)(ldY * (ns - 1) + n), true, false, &dY))) return rc;
+    if ((rc = st.map(T, sv, true, true, &dT))) return rc;
+    if (iflag && (rc = st.map(iflag, sizeof(int32_t) * (size_t)n, false, true, &dI))) return rc;
+    ohcd::ThParams P{n, (const double *)dh, (const double *)dY, ldY, (double *)dT, (int *)dI,
+                     c->opts.T_low, c->opts.T_high};
+    HIPCHK(c, c->lane->tfromh(P, c->stream));
+    return st.finish();
+}
+
+int ohc_mix_props(ohc_ctx *c, int64_t n, const double *T, const double *p, const double *Y,
+                  int64_t ldY, double *mu, double *kappa, double *Dk, int64_t ldD, double *cp,
+                  double *ha, double *hk) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    const int ns = c->mech.ns;
+    if (n < 0 || ldY < n || ((Dk || hk) && ldD < n)) return fail(c, OHC_ERR_ARG, "bad sizes");
+    if (!T || !p || !Y) return fail(c, OHC_ERR_ARG, "null array argument");
+    if ((mu || kappa || Dk) && !c->mech.has_transport)
+        return fail(c, OHC_ERR_ARG, "mechanism has no transport data");
+    if (n == 0) return OHC_OK;
+    if (!c->lane || ((mu || kappa || Dk) && !c->lane_transport))
+        return fail(c, OHC_ERR_UNSUPPORTED, "no device property kernels for this mechanism");
+    Staging st(c);
+    const size_t sv = sizeof(double) * (size_t)n, sD = sizeof(double) * (size_t)(ldD * (ns - 1) + n);
+    void *dT, *dp, *dY, *dmu, *dk, *dD, *dcp, *dha, *dhk;
+    if ((rc = st.map(T, sv, true, false, &dT))) return rc;
+    if ((rc = st.map(p, sv, true, false, &dp))) return rc;
+    if ((rc = st.map(Y, sizeof(double) * (size_t)(ldY * (ns - 1) + n), true, false, &dY))) return rc;
+    if ((rc = st.map(mu, sv, false, true, &dmu))) return rc;
+    if ((rc = st.map(kappa, sv, false, true, &dk))) return rc;
+    if ((rc = st.map(Dk, sD, false, true, &dD))) return rc;
+    if ((rc = st.map(cp, sv, false, true, &dcp))) return rc;
+    if ((rc = st.map(ha, sv, false, true, &dha))) return rc;
+    if ((rc = st.map(hk, sD, false, true, &dhk))) return rc;
+    ohcd::PropParams P{n,          (const double *)dT, (const double *)dp, (const double *)dY,
+                       ldY,        (double *)dmu,      (double *)dk,       (double *)dD,
+                       ldD,        (double *)dcp,      (double *)dha,      (double *)dhk};
+    HIPCHK(c, c->lane->props(P, c->stream));
+    return st.finish();
+}
+
+}  // extern "C"
