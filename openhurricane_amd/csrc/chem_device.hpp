@@ -77,43 +77,76 @@ struct Thermo {
     double Cp[M::NS], dCp[M::NS], H[M::NS], G[M::NS], dG[M::NS];
 };
 
-// JANAF.cpp:155-217 and thermo.inl molar wrappers for species k.
+// Coefficient set of species k at T (JANAF.inl:34-40) -- a[0..6] and the
+// host-precomputed quotients a_i/5.. (h), a_i/4.. (s), a_i/20.. (g).
 template <class M, int k>
-__device__ __forceinline__ void species_thermo(double T, double lnT, double invTm, Thermo<M> &th) {
-    const bool lo = T < M::Tcommon[k];
-    double a[7];
+struct Coef {
+    double a[7], h4, h3, h2, h1, s4, s3, s2, g4, g3, g2, g1;
+    __device__ __forceinline__ explicit Coef(double T) {
+        const bool lo = T < M::Tcommon[k];
 #pragma unroll
-    for (int c = 0; c < 7; ++c) a[c] = lo ? M::alo[k][c] : M::ahi[k][c];
-    // a[c] / {5,4,3,2} etc. precomputed on the host exactly as the reference's
-    // run-time divisions: h uses /5 /4 /3 /2, s uses /4 /3 /2, g uses /20 /12 /6 /2.
-    const double h4 = lo ? M::hlo[k][0] : M::hhi[k][0], h3 = lo ? M::hlo[k][1] : M::hhi[k][1],
-                 h2 = lo ? M::hlo[k][2] : M::hhi[k][2], h1 = lo ? M::hlo[k][3] : M::hhi[k][3];
-    const double s4 = lo ? M::slo[k][0] : M::shi[k][0], s3 = lo ? M::slo[k][1] : M::shi[k][1],
-                 s2 = lo ? M::slo[k][2] : M::shi[k][2];
-    const double g4 = lo ? M::glo[k][0] : M::ghi[k][0], g3 = lo ? M::glo[k][1] : M::ghi[k][1],
-                 g2 = lo ? M::glo[k][2] : M::ghi[k][2], g1 = lo ? M::glo[k][3] : M::ghi[k][3];
+        for (int c = 0; c < 7; ++c) a[c] = lo ? M::alo[k][c] : M::ahi[k][c];
+        h4 = lo ? M::hlo[k][0] : M::hhi[k][0];
+        h3 = lo ? M::hlo[k][1] : M::hhi[k][1];
+        h2 = lo ? M::hlo[k][2] : M::hhi[k][2];
+        h1 = lo ? M::hlo[k][3] : M::hhi[k][3];
+        s4 = lo ? M::slo[k][0] : M::shi[k][0];
+        s3 = lo ? M::slo[k][1] : M::shi[k][1];
+        s2 = lo ? M::slo[k][2] : M::shi[k][2];
+        g4 = lo ? M::glo[k][0] : M::ghi[k][0];
+        g3 = lo ? M::glo[k][1] : M::ghi[k][1];
+        g2 = lo ? M::glo[k][2] : M::ghi[k][2];
+        g1 = lo ? M::glo[k][3] : M::ghi[k][3];
+    }
+};
+
+// cp0 with the JANAF temperature clamp (JANAF.cpp:155-159), per unit mass.
+template <class M, int k>
+__device__ __forceinline__ double janaf_cp0(const Coef<M, k> &c, double T) {
+    const double Tl = M::Tlow[k], Th = M::Thigh[k];
+    const double T0 = (T < Tl || T > Th) ? fmin(fmax(T, Tl), Th) : T;
+    const double *a = c.a;
+    return ((((a[4] * T0 + a[3]) * T0 + a[2]) * T0 + a[1]) * T0 + a[0]);
+}
+
+// Molar G and dG/dT of species k (JANAF.cpp:199-212, thermo.inl:195-205).
+template <class M, int k>
+__device__ __forceinline__ void species_G(double T, double lnT, double invTm, double &G,
+                                          double &dG) {
+    const Coef<M, k> c(T);
+    const double *a = c.a;
+    const double cp0 = janaf_cp0<M, k>(c, T);
+    const double s0 = ((((c.s4 * T + c.s3) * T + c.s2) * T + a[1]) * T + a[0] * lnT + a[6]);
+    const double g0 =
+        (((((-c.g4 * T - c.g3) * T - c.g2) * T - c.g1) * T + a[0] * (1.0 - lnT) - a[6]) * T + a[5]);
+    const double ds0 = (a[0] * invTm + a[1] + (a[2] + (a[3] + a[4] * T) * T) * T);
+    constexpr double W = M::W[k];
+    G = g0 * W;
+    dG = (cp0 - (s0 + T * ds0)) * W;
+}
+
+// Molar Cp, H and dCp/dT of species k (JANAF.cpp:155-189, :214-217).
+template <class M, int k>
+__device__ __forceinline__ void species_CH(double T, double &Cp, double &H, double &dCp) {
+    const Coef<M, k> c(T);
+    const double *a = c.a;
+    const double cp0 = janaf_cp0<M, k>(c, T);
     const double Tl = M::Tlow[k], Th = M::Thigh[k];
     const bool out = (T < Tl) || (T > Th);
-    const double T0 = out ? fmin(fmax(T, Tl), Th) : T;
-    const double cp0 = ((((a[4] * T0 + a[3]) * T0 + a[2]) * T0 + a[1]) * T0 + a[0]);
-    double ha0;
-    {
-        const double Tx = out ? (T < Tl ? Tl : Th) : T;
-        double h = ((((h4 * Tx + h3) * Tx + h2) * Tx + h1) * Tx + a[0]) * Tx + a[5];
-        if (out) h += cp0 * (T - Tx);
-        ha0 = h;
-    }
-    const double s0 = ((((s4 * T + s3) * T + s2) * T + a[1]) * T + a[0] * lnT + a[6]);
-    const double g0 = (((((-g4 * T - g3) * T - g2) * T - g1) * T + a[0] * (1.0 - lnT) - a[6]) * T +
-                       a[5]);
-    const double ds0 = (a[0] * invTm + a[1] + (a[2] + (a[3] + a[4] * T) * T) * T);
+    const double Tx = out ? (T < Tl ? Tl : Th) : T;
+    double h = ((((c.h4 * Tx + c.h3) * Tx + c.h2) * Tx + c.h1) * Tx + a[0]) * Tx + a[5];
+    if (out) h += cp0 * (T - Tx);
     const double dcp = (a[1] + T * (2.0 * a[2] + T * (3.0 * a[3] + 4.0 * a[4] * T)));
     constexpr double W = M::W[k];
-    th.Cp[k] = cp0 * W;
-    th.dCp[k] = dcp * W;
-    th.H[k] = ha0 * W;
-    th.G[k] = g0 * W;
-    th.dG[k] = (cp0 - (s0 + T * ds0)) * W;
+    Cp = cp0 * W;
+    H = h * W;
+    dCp = dcp * W;
+}
+
+template <class M, int k>
+__device__ __forceinline__ void species_thermo(double T, double lnT, double invTm, Thermo<M> &th) {
+    species_G<M, k>(T, lnT, invTm, th.G[k], th.dG[k]);
+    species_CH<M, k>(T, th.Cp[k], th.H[k], th.dCp[k]);
 }
 
 // ArrheniusRR::k with the reference's skip of pow/exp when b or Ta vanish.
@@ -146,6 +179,10 @@ struct RT {
 template <class M, int r, int side, int i>
 struct Ord {
     static constexpr double value = side == 0 ? M::R[r].fo[i] : M::R[r].bo[i];
+};
+template <class M, int r, int side, int i>
+struct Nu {
+    static constexpr double value = side == 0 ? M::R[r].fnu[i] : M::R[r].bnu[i];
 };
 template <class M, int r, int side, int i>
 struct OrdM1 {
@@ -276,11 +313,36 @@ template <class M, int r, bool WITH_D>
 __device__ __forceinline__ void equilibrium(const TFac &tf, const Thermo<M> &th, double &Kc,
                                             double &dKc) {
     constexpr RxnD d = M::R[r];
-    double Kt = 0.0;
-    sfor<0, d.nf>([&](auto i) { Kt -= th.G[d.fi[i]] * d.fnu[i]; });
-    sfor<0, d.nb>([&](auto i) { Kt += th.G[d.bi[i]] * d.bnu[i]; });
-    Kt *= tf.invRuT;
-    double Kp = exp(-Kt);
+    double Kp, dk = 0.0;
+    if constexpr (M::KC_SPECIES) {
+        // th.G holds exp(-G_k/(Ru T)) and th.dG holds (dG_k - G_k/T)/(Ru T):
+        // Kp = prod_products E^nu / prod_reactants E^nu == exp(-sum nu G/(Ru T)).
+        double Pn = 1.0, Qd = 1.0;
+        sfor<0, d.nb>([&](auto i) { Pn *= cpow<Nu<M, r, 1, i>>(th.G[d.bi[i]]); });
+        sfor<0, d.nf>([&](auto i) { Qd *= cpow<Nu<M, r, 0, i>>(th.G[d.fi[i]]); });
+        Kp = Pn / Qd;
+        if constexpr (WITH_D) {
+            sfor<0, d.nf>([&](auto i) { dk -= th.dG[d.fi[i]] * d.fnu[i]; });
+            sfor<0, d.nb>([&](auto i) { dk += th.dG[d.bi[i]] * d.bnu[i]; });
+        }
+    } else {
+        double Kt = 0.0;
+        sfor<0, d.nf>([&](auto i) { Kt -= th.G[d.fi[i]] * d.fnu[i]; });
+        sfor<0, d.nb>([&](auto i) { Kt += th.G[d.bi[i]] * d.bnu[i]; });
+        Kt *= tf.invRuT;
+        Kp = exp(-Kt);
+        if constexpr (WITH_D) {
+            sfor<0, d.nf>([&](auto i) {
+                constexpr int k = d.fi[i];
+                dk -= (th.dG[k] - th.G[k] * tf.invTm) * d.fnu[i];
+            });
+            sfor<0, d.nb>([&](auto i) {
+                constexpr int k = d.bi[i];
+                dk += (th.dG[k] - th.G[k] * tf.invTm) * d.bnu[i];
+            });
+            dk *= tf.invRuT;
+        }
+    }
     if (bad(Kp)) Kp = kRootVeryLarge;
     constexpr bool zero_s = cabs(d.sumStoi) < kVeryTiny;
     double Xs = 1.0;
@@ -290,16 +352,6 @@ __device__ __forceinline__ void equilibrium(const TFac &tf, const Thermo<M> &th,
         Kc = fmin(Kp * Xs, kVeryLarge);
     }
     if constexpr (WITH_D) {
-        double dk = 0.0;
-        sfor<0, d.nf>([&](auto i) {
-            constexpr int k = d.fi[i];
-            dk -= (th.dG[k] - th.G[k] * tf.invTm) * d.fnu[i];
-        });
-        sfor<0, d.nb>([&](auto i) {
-            constexpr int k = d.bi[i];
-            dk += (th.dG[k] - th.G[k] * tf.invTm) * d.bnu[i];
-        });
-        dk *= tf.invRuT;
         double dKp = Kp * dk;
         if (bad(dKp)) dKp = kVeryLarge;
         if constexpr (zero_s) dKc = dKp;
@@ -313,13 +365,13 @@ __device__ __forceinline__ void equilibrium(const TFac &tf, const Thermo<M> &th,
     }
 }
 
-// Third-body concentration M = sum eff_k c_k (thirdBodyEfficiency::M).
-template <class M, int r>
+// Third-body concentration M = sum eff_k c_k of efficiency group g
+// (thirdBodyEfficiency::M, thirdBodyEfficiency.hpp:58-77).
+template <class M, int g>
 __device__ __forceinline__ double third_M(const double *ci) {
-    constexpr int tb = M::R[r].tb;
     double m = 0.0;
     sfor<0, M::NS>([&](auto k) {
-        constexpr double e = M::EFF[tb < 0 ? 0 : tb][k];
+        constexpr double e = M::EFF[g][k];
         if constexpr (e != 0.0) {
             if constexpr (e == 1.0) m += ci[k];
             else m += e * ci[k];
@@ -357,10 +409,25 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
     constexpr int NS = M::NS, N = NS + 1;
     const double T = y[NS];
     const TFac tf = tfac(T);
-    Thermo<M> th;
-    sfor<0, NS>([&](auto k) { species_thermo<M, k>(T, tf.lnT, tf.invTm, th); });
+    Thermo<M> th;  // G / dG now; Cp / H / dCp after the reaction loop
+    sfor<0, NS>([&](auto k) {
+        species_G<M, k>(T, tf.lnT, tf.invTm, th.G[k], th.dG[k]);
+        if constexpr (M::KC_SPECIES) {
+            const double gRT = th.G[k] * tf.invRuT;
+            th.dG[k] = (th.dG[k] - th.G[k] * tf.invTm) * tf.invRuT;
+            th.G[k] = exp(-gRT);
+        }
+    });
     double ci[NS];
     sfor<0, NS>([&](auto k) { ci[k] = fmax(y[k], 0.0); });
+    // third-body concentrations and the per-group row weights of dln(M)/dc
+    double Mg[M::NTB > 0 ? M::NTB : 1], rMg[M::NTB > 0 ? M::NTB : 1];
+    double u[M::NTB > 0 ? M::NTB : 1][NS];
+    sfor<0, M::NTB>([&](auto g) {
+        Mg[g] = third_M<M, g>(ci);
+        rMg[g] = 1.0 / fmax(Mg[g], kTiny);
+        sfor<0, NS>([&](auto a) { u[g][a] = 0.0; });
+    });
     sfor<0, N>([&](auto i) {
         f[i] = 0.0;
         if constexpr (WITH_J) sfor<0, N>([&](auto j) { J[i][j] = 0.0; });
@@ -371,7 +438,7 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
         using R = RT<M, r>;
         // ---- rate constants
         double Mtb = 0.0;
-        if constexpr (d.kind != 0) Mtb = third_M<M, r>(ci);
+        if constexpr (d.kind != 0) Mtb = Mg[d.tb];
         double kf, kr = 0.0, Kc = 0.0, dKc = 0.0, rKc = 0.0;
         FallOff<M, r> fo{};
         if constexpr (d.kind == 0) kf = arrh<0, R>(tf.lnT, tf.invT);
@@ -432,17 +499,11 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
             });
             // third-body terms q * dln(gamma)/dc_i
             if constexpr (d.kind == 1) {
-                const double rM = 1.0 / fmax(Mtb, kTiny);
-                sfor<0, NS>([&](auto i) {
-                    constexpr double e = M::EFF[d.tb][i];
-                    if constexpr (e != 0.0) {
-                        const double g = e * rM;
-                        if (!bad(g)) {
-                            sfor<0, d.nf>([&](auto j) { J[d.fi[j]][i] -= d.fnu[j] * g * q; });
-                            sfor<0, d.nb>([&](auto j) { J[d.bi[j]][i] += d.bnu[j] * g * q; });
-                        }
-                    }
-                });
+                // J(a, i) += sigma_a q eff_i / max(M, tiny): accumulate sigma_a q / M per
+                // row, the eff_i outer product is applied once per group below.
+                const double w = q * rMg[d.tb];
+                sfor<0, d.nf>([&](auto j) { u[d.tb][d.fi[j]] -= d.fnu[j] * w; });
+                sfor<0, d.nb>([&](auto j) { u[d.tb][d.bi[j]] += d.bnu[j] * w; });
             } else if constexpr (d.kind >= 2) {
                 if (Mtb > kTiny) {
                     const double kinfz = fo.kinf != 0 ? fo.kinf : kVeryTiny;
@@ -491,7 +552,21 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
             sfor<0, d.nb>([&](auto i) { J[d.bi[i]][NS] += d.bnu[i] * s; });
         }
     });
+    if constexpr (WITH_J) {
+        sfor<0, M::NTB>([&](auto g) {
+            sfor<0, NS>([&](auto a) {
+                if constexpr (M::TBROW[g][a]) {
+                    sfor<0, NS>([&](auto i) {
+                        constexpr double e = M::EFF[g][i];
+                        if constexpr (e == 1.0) J[a][i] += u[g][a];
+                        else if constexpr (e != 0.0) J[a][i] += u[g][a] * e;
+                    });
+                }
+            });
+        });
+    }
     // ---- temperature row (chemistrySource.cpp:993-1020 / :1021-1054)
+    sfor<0, NS>([&](auto k) { species_CH<M, k>(T, th.Cp[k], th.H[k], th.dCp[k]); });
     double Hx[NS], Cx[NS];
     if (flow == 0) {
         sfor<0, NS>([&](auto k) {

@@ -8,6 +8,7 @@
 // the coefficient quotients a_k/5 ... a_k/20) are computed here in double
 // exactly as the reference computes them at run time and printed with 17
 // significant digits, which round-trips every double.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <sstream>
@@ -88,24 +89,69 @@ std::string codegen_mechanism(const Mechanism &m, const std::string &name) {
     coef("shi", m.nasa_hi, 2);
     coef("glo", m.nasa_lo, 3);
     coef("ghi", m.nasa_hi, 3);
-    // third-body efficiency rows
+    // Third-body efficiency rows, one per distinct efficiency vector: the
+    // reactions of a group share M = sum eff_k c_k and their dln(M)/dc_i
+    // Jacobian terms are applied as one outer product per group.
     std::vector<int> tbrow(nr, -1);
-    int ntb = 0;
-    for (int r = 0; r < nr; ++r)
-        if (m.kind[r] != OHC_RATE_ARRHENIUS) tbrow[r] = ntb++;
+    std::vector<std::vector<double>> groups;
+    for (int r = 0; r < nr; ++r) {
+        if (m.kind[r] == OHC_RATE_ARRHENIUS) continue;
+        std::vector<double> e(m.eff.begin() + (size_t)r * ns, m.eff.begin() + (size_t)(r + 1) * ns);
+        int g = -1;
+        for (size_t q = 0; q < groups.size(); ++q)
+            if (groups[q] == e) g = (int)q;
+        if (g < 0) {
+            g = (int)groups.size();
+            groups.push_back(e);
+        }
+        tbrow[r] = g;
+    }
+    const int ntb = (int)groups.size();
+    o << "  static constexpr int NTB = " << ntb << ";\n";
     o << "  static constexpr double EFF[" << (ntb ? ntb : 1) << "][" << ns << "] = {";
     if (!ntb) {
         o << "{";
         for (int k = 0; k < ns; ++k) o << (k ? ", " : "") << "0.0";
         o << "}";
     }
-    for (int r = 0, q = 0; r < nr; ++r) {
-        if (tbrow[r] < 0) continue;
-        o << (q++ ? ", " : "") << "{";
-        for (int k = 0; k < ns; ++k) o << (k ? ", " : "") << d17(m.eff[(size_t)r * ns + k]);
+    for (int g = 0; g < ntb; ++g) {
+        o << (g ? ", " : "") << "{";
+        for (int k = 0; k < ns; ++k) o << (k ? ", " : "") << d17(groups[g][k]);
         o << "}";
     }
     o << "};\n";
+    // rows touched by the plain third-body reactions of each group
+    o << "  static constexpr bool TBROW[" << (ntb ? ntb : 1) << "][" << ns << "] = {";
+    for (int g = 0; g < std::max(ntb, 1); ++g) {
+        o << (g ? ", " : "") << "{";
+        for (int k = 0; k < ns; ++k) {
+            bool hit = false;
+            for (int r = 0; r < nr; ++r) {
+                if (tbrow[r] != g || m.kind[r] != OHC_RATE_THIRDBODY) continue;
+                for (int i = 0; i < m.nf[r]; ++i) hit |= m.f_idx[r * P + i] == k;
+                for (int i = 0; i < m.nb[r]; ++i) hit |= m.b_idx[r * P + i] == k;
+            }
+            o << (k ? ", " : "") << (hit ? "true" : "false");
+        }
+        o << "}";
+    }
+    o << "};\n";
+    // Equilibrium constants may be assembled from per-species Boltzmann factors
+    // exp(-G_k/(Ru T)) when those stay inside the double range for every
+    // temperature the integrator can see (threshold 200 K .. 6000 K).
+    bool kc_species = true;
+    for (int k = 0; k < ns && kc_species; ++k) {
+        for (double T = 200.0; T <= 6000.0; T += 10.0) {
+            const double *a = (T < m.Tcommon[k] ? &m.nasa_lo[7 * k] : &m.nasa_hi[7 * k]);
+            const double g = a[0] * (1 - std::log(T)) - a[1] * T / 2 - a[2] * T * T / 6 -
+                             a[3] * T * T * T / 12 - a[4] * T * T * T * T / 20 + a[5] / T - a[6];
+            if (!(std::fabs(g) < 600.0)) {
+                kc_species = false;
+                break;
+            }
+        }
+    }
+    o << "  static constexpr bool KC_SPECIES = " << (kc_species ? "true" : "false") << ";\n";
     o << "  static constexpr ohcd::RxnD R[" << (nr ? nr : 1) << "] = {\n";
     for (int r = 0; r < nr; ++r) {
         double s = 0.0;

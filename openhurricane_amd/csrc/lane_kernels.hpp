@@ -70,39 +70,60 @@ template <class M>
 __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
     constexpr int NS = M::NS, N = NS + 1;
     const int lane = threadIdx.x;
+    // Cold per-lane vectors live in LDS (one slot per lane, conflict-free):
+    // the step-start state y0 and the previous step-start state y_{n-1}.
+    __shared__ double s_y[N][64], s_yl[N][64], s_yk[N][64];
+    // cold scalars of the state machine and the per-lane counters
+    enum { C_DTC, C_LEFT, C_SUB, C_TN, C_TRY, C_LTS0, C_LTS1, C_ERRG, C_NCTL };
+    enum { S_CELLS, S_STEPS, S_ATT, S_NEWT, S_RHS, S_MAXS, S_FAIL, S_CSTEPS, S_CITER, S_NST };
+    __shared__ double s_c[C_NCTL][64];
+    __shared__ long long s_n[S_NST][64];
+    __shared__ double s_dtmin[64];
+#define Y0(i) s_y[i][lane]
+#define YL(i) s_yl[i][lane]
+#define YK(i) s_yk[i][lane]
+#define CT(i) s_c[i][lane]
+#define CNT(i) s_n[i][lane]
 
-    double y[N], yk[N], yl1[N], f[N], J[N][N], b[N];
+    // The Newton iterate also lives in LDS so that only f and J occupy
+    // registers through the evaluation and the factorisation.
+    double f[N], J[N][N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) y[i] = yk[i] = yl1[i] = 0.0;
-    yk[NS] = y[NS] = 1000.0;
+    for (int i = 0; i < N; ++i) {
+        YK(i) = 0.0;
+        Y0(i) = 0.0;
+        YL(i) = 0.0;
+    }
+    YK(NS) = 1000.0;
+    Y0(NS) = 1000.0;
 
     int64_t cell = -1;
     int phase = PH_NEED;
-    double pr = 0.0, dtc = 0.0, leftTime = 0.0, subDt = 0.0;
-    double tn = 0.0, t = 0.0, dt = 0.0, dtTry0 = 0.0, dtl = 0.0, lts0 = 0.0, lts1 = 0.0;
-    double lastErr = 0.0;
+    double pr = 0.0, t = 0.0, dt = 0.0, dtl = 0.0, lastErr = 0.0;
     int nstep = 0, ncount = 0, chk = 0;
     bool last = false;
-    long long st_steps = 0, st_att = 0, st_newt = 0, st_rhs = 0, st_cells = 0, st_maxs = 0,
-              st_fail = 0, cell_steps = 0, cell_iters = 0;
-    double st_dtmin = __builtin_huge_val();
-
-    const double err_grow = pow(P.maxsc / P.safe, -1.0 / P.alpha);
+#pragma unroll
+    for (int i = 0; i < C_NCTL; ++i) CT(i) = 0.0;
+#pragma unroll
+    for (int i = 0; i < S_NST; ++i) CNT(i) = 0;
+    s_dtmin[lane] = __builtin_huge_val();
+    CT(C_ERRG) = pow(P.maxsc / P.safe, -1.0 / P.alpha);
 
     // ---- state machine pieces (lane-divergent, cheap) ----
     auto finish_cell = [&]() __attribute__((always_inline)) {
+        const double subDt = CT(C_SUB);
         P.subDt[cell] = subDt;
         if (P.mode == 0) {
-            const double rho = P.rho[cell], rdt = 1.0 / dtc;
+            const double rho = P.rho[cell], rdt = 1.0 / CT(C_DTC);
             sfor<0, NS>([&](auto k) {
                 const double c0 = rho * P.Y[k * P.ldY + cell] / M::W[k];
-                P.R[k * P.ldR + cell] = (y[k] - c0) * rdt * M::W[k];
+                P.R[k * P.ldR + cell] = (Y0(k) - c0) * rdt * M::W[k];
             });
         } else {
             // chemistrySource::ciToYi(ci, yi) (chemistrySource.cpp:1222-1240)
             double yy[NS], rs = 0.0, ys = 0.0;
             sfor<0, NS>([&](auto k) {
-                yy[k] = fmax(0.0, y[k]);
+                yy[k] = fmax(0.0, Y0(k));
                 rs += yy[k] * M::W[k];
             });
             const double rr = 1.0 / fmax(rs, kTiny);
@@ -112,18 +133,18 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
             });
             const double ri = 1.0 / ys;
             sfor<0, NS>([&](auto k) { P.Yout[k * P.ldY + cell] = ys != 1.0 ? yy[k] * ri : yy[k]; });
-            P.Tout[cell] = y[NS];
+            P.Tout[cell] = Y0(NS);
         }
-        st_cells++;
-        st_dtmin = fmin(st_dtmin, subDt);
-        st_maxs = cell_steps > st_maxs ? cell_steps : st_maxs;
+        CNT(S_CELLS) += 1;
+        s_dtmin[lane] = fmin(s_dtmin[lane], subDt);
+        CNT(S_MAXS) = CNT(S_CSTEPS) > CNT(S_MAXS) ? CNT(S_CSTEPS) : CNT(S_MAXS);
         phase = PH_NEED;
     };
     auto start_attempt = [&]() __attribute__((always_inline)) {
-        lts0 = dtl;
-        st_att++;
+        CT(C_LTS0) = dtl;
+        CNT(S_ATT) += 1;
 #pragma unroll
-        for (int i = 0; i < N; ++i) yk[i] = y[i];
+        for (int i = 0; i < N; ++i) YK(i) = Y0(i);
         ncount = 0;
         lastErr = 0.0;
         phase = PH_NEWTON;
@@ -131,11 +152,12 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
     // ImplicitEuler::solve(t0, tn, dt0, y) loop head for step nstep; false when
     // the step budget is exhausted (tn then becomes the time reached).
     auto start_step = [&]() __attribute__((always_inline)) -> bool {
+        const double tn = CT(C_TN);
         if (nstep >= P.max_steps) {
-            tn = t;
+            CT(C_TN) = t;
             return false;
         }
-        dtTry0 = dt;
+        CT(C_TRY) = dt;
         if ((t + dt - tn) * (t + dt) > 0) {
             dt = tn - t;
             last = true;
@@ -143,26 +165,27 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
             last = true;
         }
         dtl = dt;  // adaptiveSolve entry
-        lts1 = lts0;
+        CT(C_LTS1) = CT(C_LTS0);
         chk = 0;
         start_attempt();
         return true;
     };
     auto start_ode = [&]() __attribute__((always_inline)) {
-        tn = leftTime;
+        CT(C_TN) = CT(C_LEFT);
         t = 0.0;
-        dt = 0.2 * subDt;
+        dt = 0.2 * CT(C_SUB);
         last = false;
         nstep = 0;
         return start_step();
     };
-    // ImplicitEuler::maxIncrement
+    // ImplicitEuler::maxIncrement of the error vector e (held in f)
     auto max_increment = [&](const double(&e)[N]) __attribute__((always_inline)) {
         double m = 0.0;
         bool nan_y = false;
         sfor<0, N>([&](auto i) {
-            nan_y |= isnan(yk[i]);
-            const double tol = P.atol_inc + P.rtol_inc * fmax(fabs(y[i]), fabs(yk[i]));
+            const double yki = YK(i);
+            nan_y |= isnan(yki);
+            const double tol = P.atol_inc + P.rtol_inc * fmax(fabs(Y0(i)), fabs(yki));
             m = fmax(m, fabs(e[i]) / tol);
         });
         return nan_y ? 1.5 : m;
@@ -172,7 +195,8 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
     enum { A_RETRY, A_FAIL, A_NEXT, A_ODE_END };
     auto attempt_done = [&](double error) __attribute__((always_inline)) {
         int action;
-        const bool ok = !((yk[NS] - y[NS]) > P.Talpha * y[NS]);
+        const double y0T = Y0(NS);
+        const bool ok = !((YK(NS) - y0T) > P.Talpha * y0T);
         if (!ok) dtl *= P.Tbeta;
         if (!ok && chk < 100) {
             chk++;
@@ -184,17 +208,18 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
             t += dtl;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-                yl1[i] = y[i];
-                y[i] = yk[i];
+                YL(i) = Y0(i);
+                Y0(i) = YK(i);
             }
-            if (error > err_grow)
+            if (error > CT(C_ERRG))
                 dt = fmin(fmax(P.safe * pow(error, -P.alpha), P.minsc), P.maxsc) * dtl;
             else
                 dt = fmax(P.safe * P.maxsc * dtl, dtl);
-            st_steps++;
-            cell_steps++;
+            CNT(S_STEPS) += 1;
+            CNT(S_CSTEPS) += 1;
+            const double tn = CT(C_TN);
             if ((t - tn) * tn >= 0) {
-                if (nstep > 0 && last) subDt = fmin(dtTry0, tn);
+                if (nstep > 0 && last) CT(C_SUB) = fmin(CT(C_TRY), tn);
                 action = A_ODE_END;
             } else {
                 nstep++;
@@ -208,18 +233,20 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
         }
         if (action == A_ODE_END) {
             // chemistrySource::solve: while (leftTime > 0) { dt = leftTime; solve(c, dt, subDt); leftTime -= dt; }
-            leftTime -= tn;
-            if (leftTime > 0.0) start_ode();
+            const double left = CT(C_LEFT) - CT(C_TN);
+            CT(C_LEFT) = left;
+            if (left > 0.0) start_ode();
             else finish_cell();
         } else if (action == A_FAIL) {
-            st_fail++;
+            CNT(S_FAIL) += 1;
             finish_cell();
         }
     };
     // start a fresh cell (chemistrySource::solve / solveUpdate loop body)
     auto start_cell = [&]() __attribute__((always_inline)) {
         const double T0 = P.T[cell];
-        dtc = P.dt[cell] * P.dtFactor;
+        const double dtc = P.dt[cell] * P.dtFactor;
+        CT(C_DTC) = dtc;
         const double dtI = P.subDt[cell] > 0.0 ? P.subDt[cell] : dtc;
         if (T0 < P.Tthr) {
             P.subDt[cell] = dtc;
@@ -228,12 +255,12 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
         }
         const double rho = P.rho[cell];
         pr = P.flow == 0 ? P.p[cell] : rho;
-        sfor<0, NS>([&](auto k) { y[k] = rho * P.Y[k * P.ldY + cell] / M::W[k]; });
-        y[NS] = T0;
-        subDt = fmin(dtc, fmin(dtI, P.dt_max));
-        leftTime = dtc;
-        cell_steps = 0;
-        cell_iters = 0;
+        sfor<0, NS>([&](auto k) { Y0(k) = rho * P.Y[k * P.ldY + cell] / M::W[k]; });
+        Y0(NS) = T0;
+        CT(C_SUB) = fmin(dtc, fmin(dtI, P.dt_max));
+        CT(C_LEFT) = dtc;
+        CNT(S_CSTEPS) = 0;
+        CNT(S_CITER) = 0;
         start_ode();
     };
 
@@ -259,29 +286,42 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
         }
         if (!__any(phase == PH_NEWTON || phase == PH_ERR0)) break;
 
-        // ---- the wavefront-wide heavy step ----
-        eval_fJ<M, true>(yk, pr, P.flow, f, J);
+        // ---- the wavefront-wide heavy step: f and J at the Newton iterate ----
+        {
+            double yk[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) yk[i] = YK(i);
+            eval_fJ<M, true>(yk, pr, P.flow, f, J);
+        }
+
+        bool have = false;
+        double aerr = 0.0;
+        if (phase == PH_ERR0) {
+            // first step: e = y - y0 - dt f(y) (ImplicitEuler.cpp:111-120)
+            CNT(S_RHS) += 1;
+            sfor<0, N>([&](auto i) { f[i] = YK(i) - Y0(i) - dtl * f[i]; });
+            have = true;
+            aerr = max_increment(f);
+        }
         if (__any(phase == PH_NEWTON)) {
             // (I - dt J) delta = -yk + y0 + dt f  (ImplicitEuler.cpp:186-198)
             sfor<0, N>([&](auto i) {
                 sfor<0, N>([&](auto j) { J[i][j] = -dtl * J[i][j]; });
                 J[i][i] += 1.0;
-                b[i] = -yk[i] + y[i] + dtl * f[i];
+                f[i] = -YK(i) + Y0(i) + dtl * f[i];
             });
-            lu_solve<N>(J, b);
+            lu_solve<N>(J, f);
         }
 
         // ---- lane-local transitions ----
-        bool have = false;
-        double aerr = 0.0;
         if (phase == PH_NEWTON) {
-            // NewtonIteration body after the solve (ImplicitEuler.cpp:200-249)
-            st_newt++;
+            // NewtonIteration body after the solve (ImplicitEuler.cpp:200-249); delta in f
+            CNT(S_NEWT) += 1;
             bool stop = false;
             sfor<0, N>([&](auto i) {
                 if (!stop) {
-                    if (bad(b[i])) stop = true;
-                    else yk[i] += b[i];
+                    if (bad(f[i])) stop = true;
+                    else YK(i) += f[i];
                 }
             });
             if (stop) {
@@ -290,8 +330,8 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
             } else {
                 double maxErr = 0.0;
                 sfor<0, N>([&](auto i) {
-                    const double tol = P.atol + P.rtol * fabs(yk[i]);
-                    maxErr = fmax(maxErr, fabs(b[i]) / tol);
+                    const double tol = P.atol + P.rtol * fabs(YK(i));
+                    maxErr = fmax(maxErr, fabs(f[i]) / tol);
                 });
                 const bool diverged =
                     ncount > 0 && (maxErr / fmax(lastErr, kTiny)) > 2 && ncount > 5;
@@ -313,41 +353,39 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
                         phase = PH_ERR0;
                     } else {
                         // second-difference LTE (ImplicitEuler.cpp:122-139)
-                        const double dtn = lts0, dtn1 = lts1;
+                        const double dtn = CT(C_LTS0), dtn1 = CT(C_LTS1);
                         const double f1 = 2.0 / (dtn * (dtn + dtn1));
                         const double f2 = 2.0 / (dtn1 * (dtn + dtn1));
                         const double LTEF = dtn * dtn * 0.5;
-                        double e[N];
                         sfor<0, N>([&](auto i) {
-                            e[i] = (f1 * (yk[i] - y[i]) - f2 * (y[i] - yl1[i])) * LTEF;
+                            const double y0 = Y0(i);
+                            f[i] = (f1 * (YK(i) - y0) - f2 * (y0 - YL(i))) * LTEF;
                         });
                         have = true;
-                        aerr = max_increment(e);
+                        aerr = max_increment(f);
                     }
                 }
             }
-        } else if (phase == PH_ERR0) {
-            // first step: e = y - y0 - dt f(y) (ImplicitEuler.cpp:111-120)
-            st_rhs++;
-            double e[N];
-            sfor<0, N>([&](auto i) { e[i] = yk[i] - y[i] - dtl * f[i]; });
-            have = true;
-            aerr = max_increment(e);
         }
         if (have) {
             attempt_done(aerr);
-        } else if (phase == PH_NEWTON && ++cell_iters > P.max_cell_iters) {
+        } else if (phase == PH_NEWTON && (CNT(S_CITER) += 1) > P.max_cell_iters) {
             // no legitimate cell gets here; the cap guarantees every wavefront drains
-            st_fail++;
+            CNT(S_FAIL) += 1;
             finish_cell();
         }
     }
-
     // ---- counters: one atomic per wavefront ----
-    const long long s0 = wave_sum(st_cells), s1 = wave_sum(st_steps), s2 = wave_sum(st_att),
-                    s3 = wave_sum(st_newt), s4 = wave_sum(st_rhs), s5 = wave_max(st_maxs),
-                    s7 = wave_sum(st_fail);
-    const double dmin = wave_min(st_dtmin);
+    const long long s0 = wave_sum(CNT(S_CELLS)), s1 = wave_sum(CNT(S_STEPS)),
+                    s2 = wave_sum(CNT(S_ATT)), s3 = wave_sum(CNT(S_NEWT)),
+                    s4 = wave_sum(CNT(S_RHS)), s5 = wave_max(CNT(S_MAXS)),
+                    s7 = wave_sum(CNT(S_FAIL));
+    const double dmin = wave_min(s_dtmin[lane]);
+#undef Y0
+#undef YL
+#undef YK
+#undef CT
+#undef CNT
     if (lane == 0) {
         atomicAdd(&P.stats[0], (unsigned long long)s0);
         atomicAdd(&P.stats[1], (unsigned long long)s1);
