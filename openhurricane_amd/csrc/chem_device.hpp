@@ -60,6 +60,19 @@ __device__ __forceinline__ void sfor(F &&f) {
 
 __device__ __forceinline__ bool bad(double x) { return isnan(x) || isinf(x); }
 
+// 1/x from the hardware reciprocal estimate refined by two Newton steps
+// (within 1 ulp of the correctly rounded quotient for normal x; 0 -> inf,
+// inf -> 0, NaN -> NaN).  Used where a quotient feeds a tolerance ratio, a
+// pivot or a Kc reciprocal, never for a value that is compared bit-exactly.
+__device__ __forceinline__ double frcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    if (isinf(x) || x == 0.0) return r;
+    double e = fma(-x, r, 1.0);
+    r = fma(e, r, r);
+    e = fma(-x, r, 1.0);
+    return fma(e, r, r);
+}
+
 // x^p for a compile-time exponent p >= 0 (integer exponents as products).
 template <class P>
 __device__ __forceinline__ double cpow(double x) {
@@ -284,9 +297,9 @@ __device__ __forceinline__ TFac tfac(double T) {
     f.T = T;
     const double Tm = fmax(T, kTiny);
     f.lnT = log(Tm);
-    f.invT = 1.0 / T;
-    f.invTm = 1.0 / Tm;
-    f.invRuT = 1.0 / (Tm * kRu);
+    f.invT = frcp(T);
+    f.invTm = frcp(Tm);
+    f.invRuT = frcp(Tm * kRu);
     f.X = kPatm / (kRu * T);  // Patm / (Ru T), the Kc conversion base
     f.Xinv = (kRu * T) / kPatm;
     return f;
@@ -309,21 +322,41 @@ struct SumS {
 // Equilibrium terms of reversible reaction r: Kc, 1/max(Kc, veryTiny) and
 // (optionally) dKc/dT -- reactionTypes.cpp:178-241 (dKp/dT with the
 // reference's sign convention).
+template <class M, int r>
+struct MSumS {
+    static constexpr double value = -M::R[r].sumStoi;
+};
+
+// Equilibrium terms of reversible reaction r: Kc, rK = 1/max(Kc, veryTiny)
+// and (optionally) dKc/dT -- reactionTypes.cpp:178-241 (dKp/dT with the
+// reference's sign convention).
 template <class M, int r, bool WITH_D>
 __device__ __forceinline__ void equilibrium(const TFac &tf, const Thermo<M> &th, double &Kc,
-                                            double &dKc) {
+                                            double &rK, double &dKc) {
     constexpr RxnD d = M::R[r];
-    double Kp, dk = 0.0;
+    constexpr bool zero_s = cabs(d.sumStoi) < kVeryTiny;
+    double Kp, rKp = 0.0, dk = 0.0;
     if constexpr (M::KC_SPECIES) {
-        // th.G holds exp(-G_k/(Ru T)) and th.dG holds (dG_k - G_k/T)/(Ru T):
-        // Kp = prod_products E^nu / prod_reactants E^nu == exp(-sum nu G/(Ru T)).
-        double Pn = 1.0, Qd = 1.0;
-        sfor<0, d.nb>([&](auto i) { Pn *= cpow<Nu<M, r, 1, i>>(th.G[d.bi[i]]); });
-        sfor<0, d.nf>([&](auto i) { Qd *= cpow<Nu<M, r, 0, i>>(th.G[d.fi[i]]); });
-        Kp = Pn / Qd;
+        // th.G = E_k = exp(-G_k/(Ru T)), th.H = 1/E_k, th.dG = (dG_k - G_k/T)/(Ru T):
+        // Kp = prod_products E^nu prod_reactants (1/E)^nu == exp(-sum nu G/(Ru T)).
+        double Pn = 1.0, Qd = 1.0, Pi = 1.0, Qi = 1.0;
+        sfor<0, d.nb>([&](auto i) {
+            Pn *= cpow<Nu<M, r, 1, i>>(th.G[d.bi[i]]);
+            Pi *= cpow<Nu<M, r, 1, i>>(th.H[d.bi[i]]);
+        });
+        sfor<0, d.nf>([&](auto i) {
+            Qd *= cpow<Nu<M, r, 0, i>>(th.H[d.fi[i]]);
+            Qi *= cpow<Nu<M, r, 0, i>>(th.G[d.fi[i]]);
+        });
+        Kp = Pn * Qd;
+        rKp = Pi * Qi;
         if constexpr (WITH_D) {
             sfor<0, d.nf>([&](auto i) { dk -= th.dG[d.fi[i]] * d.fnu[i]; });
             sfor<0, d.nb>([&](auto i) { dk += th.dG[d.bi[i]] * d.bnu[i]; });
+        }
+        if (bad(Kp)) {
+            Kp = kRootVeryLarge;
+            rKp = 1.0 / kRootVeryLarge;
         }
     } else {
         double Kt = 0.0;
@@ -342,14 +375,21 @@ __device__ __forceinline__ void equilibrium(const TFac &tf, const Thermo<M> &th,
             });
             dk *= tf.invRuT;
         }
+        if (bad(Kp)) Kp = kRootVeryLarge;
     }
-    if (bad(Kp)) Kp = kRootVeryLarge;
-    constexpr bool zero_s = cabs(d.sumStoi) < kVeryTiny;
     double Xs = 1.0;
-    if constexpr (zero_s) Kc = Kp;
-    else {
+    if constexpr (zero_s) {
+        Kc = Kp;
+    } else {
         Xs = Xpow<SumS<M, r>>(tf);
         Kc = fmin(Kp * Xs, kVeryLarge);
+    }
+    if constexpr (M::KC_SPECIES) {
+        double rKc = rKp;
+        if constexpr (!zero_s) rKc = Kc == kVeryLarge ? 1.0 / kVeryLarge : rKp * Xpow<MSumS<M, r>>(tf);
+        rK = Kc > kVeryTiny ? rKc : 1.0 / kVeryTiny;
+    } else {
+        rK = frcp(fmax(Kc, kVeryTiny));
     }
     if constexpr (WITH_D) {
         double dKp = Kp * dk;
@@ -400,10 +440,10 @@ __device__ __forceinline__ FallOff<M, r> falloff(const TFac &tf, double Mtb) {
 }
 
 // Evaluate dy/dt (and the Jacobian when WITH_J) at y = [c_1..c_ns, T] for the
-// constant-pressure reactor (flow == 0) or constant volume (flow == 1, pr =
+// constant-pressure reactor (FLOW == 0) or constant volume (FLOW == 1, pr =
 // rho0).  f has NS+1 entries, J is (NS+1)x(NS+1) row-major in registers.
-template <class M, bool WITH_J>
-__device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr, int flow,
+template <class M, bool WITH_J, int FLOW>
+__device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
                                         double (&f)[M::NS + 1],
                                         double (&J)[M::NS + 1][M::NS + 1]) {
     constexpr int NS = M::NS, N = NS + 1;
@@ -416,6 +456,7 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
             const double gRT = th.G[k] * tf.invRuT;
             th.dG[k] = (th.dG[k] - th.G[k] * tf.invTm) * tf.invRuT;
             th.G[k] = exp(-gRT);
+            th.H[k] = frcp(th.G[k]);  // exp(+G/(Ru T)), slot reused until the T row
         }
     });
     double ci[NS];
@@ -449,8 +490,7 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
             else kf = fo.k0 * (1.0 / (1.0 + fo.Pr)) * fo.F;
         }
         if constexpr (d.rev == 1) {
-            equilibrium<M, r, WITH_J>(tf, th, Kc, dKc);
-            rKc = 1.0 / fmax(Kc, kVeryTiny);
+            equilibrium<M, r, WITH_J>(tf, th, Kc, rKc, dKc);
             kr = Kc > kVeryTiny ? kf * rKc : 0.0;
         } else if constexpr (d.rev == 2) {
             kr = arrh<2, R>(tf.lnT, tf.invT);
@@ -568,16 +608,15 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
     // ---- temperature row (chemistrySource.cpp:993-1020 / :1021-1054)
     sfor<0, NS>([&](auto k) { species_CH<M, k>(T, th.Cp[k], th.H[k], th.dCp[k]); });
     double Hx[NS], Cx[NS];
-    if (flow == 0) {
+    if constexpr (FLOW == 0) {
         sfor<0, NS>([&](auto k) {
             Hx[k] = th.H[k];
             Cx[k] = th.Cp[k];
         });
     } else {
         sfor<0, NS>([&](auto k) {
-            constexpr double Wk = M::W[k], Rk = M::Ri[k];
-            Hx[k] = (th.H[k] / Wk - Rk * T) * Wk;  // ea * W
-            Cx[k] = (th.Cp[k] / Wk - Rk) * Wk;     // Cv
+            Hx[k] = th.H[k] - kRu * T;  // ea W = (ha - Ri T) W
+            Cx[k] = th.Cp[k] - kRu;     // Cv W = (cp - Ri) W
         });
     }
     double cC = 0.0, dT = 0.0, dcC = 0.0;
@@ -586,7 +625,7 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
         if constexpr (WITH_J) dcC += y[k] * th.dCp[k];
         dT += f[k] * Hx[k];
     });
-    const double rcC = 1.0 / cC;
+    const double rcC = frcp(cC);
     dT *= rcC;
     f[NS] = -dT;
     if constexpr (WITH_J) {
@@ -610,6 +649,7 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
 // needs them.
 template <int N>
 __device__ __forceinline__ void lu_solve(double (&A)[N][N], double (&b)[N]) {
+    double dinv[N];
     sfor<0, N>([&](auto kk) {
         constexpr int k = kk;
         double s[N];
@@ -649,10 +689,8 @@ __device__ __forceinline__ void lu_solve(double (&A)[N][N], double (&b)[N]) {
             sfor<0, k>([&](auto t) { sum += A[k][t] * A[t][j]; });
             A[k][j] -= sum;
         });
-        if constexpr (k < N - 1) {
-            const double inv = 1.0 / A[k][k];
-            sfor<k + 1, N>([&](auto i) { A[i][k] = s[i] * inv; });
-        }
+        dinv[k] = frcp(A[k][k]);
+        sfor<k + 1, N>([&](auto i) { A[i][k] = s[i] * dinv[k]; });
     });
     sfor<0, N>([&](auto i) {
         double sum = 0.0;
@@ -663,7 +701,7 @@ __device__ __forceinline__ void lu_solve(double (&A)[N][N], double (&b)[N]) {
         constexpr int i = N - 1 - ii;
         double sum = 0.0;
         sfor<i + 1, N>([&](auto t) { sum += A[i][t] * b[t]; });
-        b[i] = (b[i] - sum) / A[i][i];
+        b[i] = (b[i] - sum) * dinv[i];
     });
 }
 

@@ -66,7 +66,7 @@ __device__ __forceinline__ long long wave_max(long long v) {
     return v;
 }
 
-template <class M>
+template <class M, int FLOW>
 __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
     constexpr int NS = M::NS, N = NS + 1;
     const int lane = threadIdx.x;
@@ -74,11 +74,36 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
     // the step-start state y0 and the previous step-start state y_{n-1}.
     __shared__ double s_y[N][64], s_yl[N][64], s_yk[N][64];
     // cold scalars of the state machine and the per-lane counters
-    enum { C_DTC, C_LEFT, C_SUB, C_TN, C_TRY, C_LTS0, C_LTS1, C_ERRG, C_NCTL };
-    enum { S_CELLS, S_STEPS, S_ATT, S_NEWT, S_RHS, S_MAXS, S_FAIL, S_CSTEPS, S_CITER, S_NST };
+    enum { C_DTC, C_LEFT, C_SUB, C_TN, C_TRY, C_LTS0, C_LTS1, C_PR, C_T, C_DT, C_DTL, C_LERR,
+           C_NCTL };
+    enum { S_CELLS, S_STEPS, S_ATT, S_NEWT, S_RHS, S_MAXS, S_FAIL, S_CSTEPS, S_CITER, S_CELL,
+           S_NSTEP, S_NCOUNT, S_CHK, S_LAST, S_NST };
     __shared__ double s_c[C_NCTL][64];
     __shared__ long long s_n[S_NST][64];
     __shared__ double s_dtmin[64];
+    // integrator parameters, uniform across the block (broadcast LDS reads keep
+    // them out of the scalar register file)
+    enum { Q_ATOL, Q_RTOL, Q_ATOLI, Q_RTOLI, Q_SAFE, Q_MINSC, Q_MAXSC, Q_ALPHA, Q_TTHR, Q_TALPHA,
+           Q_TBETA, Q_DTMAX, Q_DTF, Q_ERRG, Q_NPAR };
+    __shared__ double s_par[Q_NPAR];
+    if (lane == 0) {
+        s_par[Q_ATOL] = P.atol;
+        s_par[Q_RTOL] = P.rtol;
+        s_par[Q_ATOLI] = P.atol_inc;
+        s_par[Q_RTOLI] = P.rtol_inc;
+        s_par[Q_SAFE] = P.safe;
+        s_par[Q_MINSC] = P.minsc;
+        s_par[Q_MAXSC] = P.maxsc;
+        s_par[Q_ALPHA] = P.alpha;
+        s_par[Q_TTHR] = P.Tthr;
+        s_par[Q_TALPHA] = P.Talpha;
+        s_par[Q_TBETA] = P.Tbeta;
+        s_par[Q_DTMAX] = P.dt_max;
+        s_par[Q_DTF] = P.dtFactor;
+        s_par[Q_ERRG] = pow(P.maxsc / P.safe, -1.0 / P.alpha);
+    }
+    __syncthreads();
+#define PAR(i) s_par[i]
 #define Y0(i) s_y[i][lane]
 #define YL(i) s_yl[i][lane]
 #define YK(i) s_yk[i][lane]
@@ -97,17 +122,19 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
     YK(NS) = 1000.0;
     Y0(NS) = 1000.0;
 
-    int64_t cell = -1;
+    // All loop-carried lane state except the phase lives in LDS: the loop body
+    // then holds only f, J and transients in registers.
     int phase = PH_NEED;
-    double pr = 0.0, t = 0.0, dt = 0.0, dtl = 0.0, lastErr = 0.0;
-    int nstep = 0, ncount = 0, chk = 0;
-    bool last = false;
 #pragma unroll
     for (int i = 0; i < C_NCTL; ++i) CT(i) = 0.0;
 #pragma unroll
     for (int i = 0; i < S_NST; ++i) CNT(i) = 0;
     s_dtmin[lane] = __builtin_huge_val();
-    CT(C_ERRG) = pow(P.maxsc / P.safe, -1.0 / P.alpha);
+    long long &cell = CNT(S_CELL);
+    long long &nstep = CNT(S_NSTEP), &ncount = CNT(S_NCOUNT), &chk = CNT(S_CHK),
+              &last = CNT(S_LAST);
+    double &pr = CT(C_PR), &t = CT(C_T), &dt = CT(C_DT), &dtl = CT(C_DTL),
+           &lastErr = CT(C_LERR);
 
     // ---- state machine pieces (lane-divergent, cheap) ----
     auto finish_cell = [&]() __attribute__((always_inline)) {
@@ -185,8 +212,8 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
         sfor<0, N>([&](auto i) {
             const double yki = YK(i);
             nan_y |= isnan(yki);
-            const double tol = P.atol_inc + P.rtol_inc * fmax(fabs(Y0(i)), fabs(yki));
-            m = fmax(m, fabs(e[i]) / tol);
+            const double tol = PAR(Q_ATOLI) + PAR(Q_RTOLI) * fmax(fabs(Y0(i)), fabs(yki));
+            m = fmax(m, fabs(e[i]) * frcp(tol));
         });
         return nan_y ? 1.5 : m;
     };
@@ -196,13 +223,13 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
     auto attempt_done = [&](double error) __attribute__((always_inline)) {
         int action;
         const double y0T = Y0(NS);
-        const bool ok = !((YK(NS) - y0T) > P.Talpha * y0T);
-        if (!ok) dtl *= P.Tbeta;
+        const bool ok = !((YK(NS) - y0T) > PAR(Q_TALPHA) * y0T);
+        if (!ok) dtl *= PAR(Q_TBETA);
         if (!ok && chk < 100) {
             chk++;
             action = dtl <= kVeryTiny ? A_FAIL : A_RETRY;
         } else if (error > 1.0) {
-            dtl *= fmax(P.safe * pow(error, -P.alpha), P.minsc);
+            dtl *= fmax(PAR(Q_SAFE) * pow(error, -PAR(Q_ALPHA)), PAR(Q_MINSC));
             action = dtl <= kVeryTiny ? A_FAIL : A_RETRY;
         } else {
             t += dtl;
@@ -211,10 +238,10 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
                 YL(i) = Y0(i);
                 Y0(i) = YK(i);
             }
-            if (error > CT(C_ERRG))
-                dt = fmin(fmax(P.safe * pow(error, -P.alpha), P.minsc), P.maxsc) * dtl;
+            if (error > PAR(Q_ERRG))
+                dt = fmin(fmax(PAR(Q_SAFE) * pow(error, -PAR(Q_ALPHA)), PAR(Q_MINSC)), PAR(Q_MAXSC)) * dtl;
             else
-                dt = fmax(P.safe * P.maxsc * dtl, dtl);
+                dt = fmax(PAR(Q_SAFE) * PAR(Q_MAXSC) * dtl, dtl);
             CNT(S_STEPS) += 1;
             CNT(S_CSTEPS) += 1;
             const double tn = CT(C_TN);
@@ -245,10 +272,10 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
     // start a fresh cell (chemistrySource::solve / solveUpdate loop body)
     auto start_cell = [&]() __attribute__((always_inline)) {
         const double T0 = P.T[cell];
-        const double dtc = P.dt[cell] * P.dtFactor;
+        const double dtc = P.dt[cell] * PAR(Q_DTF);
         CT(C_DTC) = dtc;
         const double dtI = P.subDt[cell] > 0.0 ? P.subDt[cell] : dtc;
-        if (T0 < P.Tthr) {
+        if (T0 < PAR(Q_TTHR)) {
             P.subDt[cell] = dtc;
             if (P.mode == 0) sfor<0, NS>([&](auto k) { P.R[k * P.ldR + cell] = 0.0; });
             return;  // stays PH_NEED
@@ -257,7 +284,7 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
         pr = P.flow == 0 ? P.p[cell] : rho;
         sfor<0, NS>([&](auto k) { Y0(k) = rho * P.Y[k * P.ldY + cell] / M::W[k]; });
         Y0(NS) = T0;
-        CT(C_SUB) = fmin(dtc, fmin(dtI, P.dt_max));
+        CT(C_SUB) = fmin(dtc, fmin(dtI, PAR(Q_DTMAX)));
         CT(C_LEFT) = dtc;
         CNT(S_CSTEPS) = 0;
         CNT(S_CITER) = 0;
@@ -279,7 +306,7 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
                 if (mine >= (unsigned long long)P.n) {
                     phase = PH_DONE;
                 } else {
-                    cell = (int64_t)mine;
+                    cell = (long long)mine;
                     start_cell();
                 }
             }
@@ -287,11 +314,16 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
         if (!__any(phase == PH_NEWTON || phase == PH_ERR0)) break;
 
         // ---- the wavefront-wide heavy step: f and J at the Newton iterate ----
+        // The fences keep the compiler from hoisting control-state loads above
+        // (or sinking them below) the evaluation and factorisation, so only f,
+        // J and their transients are live there.
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
         {
             double yk[N];
 #pragma unroll
             for (int i = 0; i < N; ++i) yk[i] = YK(i);
-            eval_fJ<M, true>(yk, pr, P.flow, f, J);
+            eval_fJ<M, true, FLOW>(yk, pr, f, J);
         }
 
         bool have = false;
@@ -305,13 +337,16 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
         }
         if (__any(phase == PH_NEWTON)) {
             // (I - dt J) delta = -yk + y0 + dt f  (ImplicitEuler.cpp:186-198)
+            const double h = dtl;
             sfor<0, N>([&](auto i) {
-                sfor<0, N>([&](auto j) { J[i][j] = -dtl * J[i][j]; });
+                sfor<0, N>([&](auto j) { J[i][j] = -h * J[i][j]; });
                 J[i][i] += 1.0;
-                f[i] = -YK(i) + Y0(i) + dtl * f[i];
+                f[i] = -YK(i) + Y0(i) + h * f[i];
             });
             lu_solve<N>(J, f);
         }
+        __builtin_amdgcn_sched_barrier(0);
+        asm volatile("" ::: "memory");
 
         // ---- lane-local transitions ----
         if (phase == PH_NEWTON) {
@@ -330,8 +365,8 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
             } else {
                 double maxErr = 0.0;
                 sfor<0, N>([&](auto i) {
-                    const double tol = P.atol + P.rtol * fabs(YK(i));
-                    maxErr = fmax(maxErr, fabs(f[i]) / tol);
+                    const double tol = PAR(Q_ATOL) + PAR(Q_RTOL) * fabs(YK(i));
+                    maxErr = fmax(maxErr, fabs(f[i]) * frcp(tol));
                 });
                 const bool diverged =
                     ncount > 0 && (maxErr / fmax(lastErr, kTiny)) > 2 && ncount > 5;
@@ -386,6 +421,7 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
 #undef YK
 #undef CT
 #undef CNT
+#undef PAR
     if (lane == 0) {
         atomicAdd(&P.stats[0], (unsigned long long)s0);
         atomicAdd(&P.stats[1], (unsigned long long)s1);
@@ -419,7 +455,8 @@ __global__ __launch_bounds__(64) void eval_lane_kernel(const EvalParams P) {
     if (cell >= P.n) return;
     double y[N], f[N], J[N][N];
     sfor<0, N>([&](auto i) { y[i] = P.y[i * P.ldy + cell]; });
-    eval_fJ<M, true>(y, P.pr[cell], P.flow, f, J);
+    if (P.flow == 0) eval_fJ<M, true, 0>(y, P.pr[cell], f, J);
+    else eval_fJ<M, true, 1>(y, P.pr[cell], f, J);
     sfor<0, N>([&](auto i) { P.f[i * P.ldf + cell] = f[i]; });
     if (P.J)
         sfor<0, N>([&](auto i) {
@@ -591,7 +628,8 @@ struct LaneEntry {
 template <class M>
 struct LaneLaunch {
     static hipError_t integrate(const LaneParams &p, int grid, hipStream_t s) {
-        hipLaunchKernelGGL(ie_lane_kernel<M>, dim3(grid), dim3(64), 0, s, p);
+        if (p.flow == 0) hipLaunchKernelGGL((ie_lane_kernel<M, 0>), dim3(grid), dim3(64), 0, s, p);
+        else hipLaunchKernelGGL((ie_lane_kernel<M, 1>), dim3(grid), dim3(64), 0, s, p);
         return hipGetLastError();
     }
     static hipError_t eval(const EvalParams &p, hipStream_t s) {
@@ -611,7 +649,7 @@ struct LaneLaunch {
     }
     static LaneEntry entry(const char *name) {
         return LaneEntry{name,      M::HASH,      M::THASH, M::NS, &integrate, &eval,
-                         &props,    &tfromh,      (const void *)&ie_lane_kernel<M>};
+                         &props,    &tfromh,      (const void *)&ie_lane_kernel<M, 0>};
     }
 };
 
