@@ -141,6 +141,10 @@ typedef struct ohc_opts {
     int32_t device_pointers;/* 1: array arguments are device pointers             */
     int32_t kernel;         /* 0 auto, 1 lane-per-cell specialised, 2 generic     */
     int32_t waves_per_cu;   /* persistent-kernel occupancy target (0 = auto)      */
+    int32_t schedule;       /* 0: cells in index order; 1: longest-first, ordered
+                               by the per-cell Newton-iteration counts of the
+                               previous call with the same n (first call: index
+                               order).  Any order gives the same results.         */
 } ohc_opts;
 
 /* Per-call counters (summed over cells) for roofline accounting. */

@@ -44,6 +44,8 @@ struct LaneParams {
     double atol, rtol, atol_inc, rtol_inc, safe, minsc, maxsc, alpha;
     double Tthr, Talpha, Tbeta, dt_max;
     long long max_cell_iters;     // safety cap on Newton iterations per cell (termination)
+    const int *order;             // queue order (a permutation of 0..n-1) or null
+    int *cost;                    // per-cell Newton iterations of this call, or null
     unsigned long long *counter;  // work-queue head, zeroed before launch
     unsigned long long *stats;    // see ohc_stats: [0..5] counters, [6] dt_min bits, [7] failures
 };
@@ -163,6 +165,7 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
             P.Tout[cell] = Y0(NS);
         }
         CNT(S_CELLS) += 1;
+        if (P.cost) P.cost[cell] = (int)CNT(S_CITER);
         s_dtmin[lane] = fmin(s_dtmin[lane], subDt);
         CNT(S_MAXS) = CNT(S_CSTEPS) > CNT(S_MAXS) ? CNT(S_CSTEPS) : CNT(S_MAXS);
         phase = PH_NEED;
@@ -277,6 +280,7 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
         const double dtI = P.subDt[cell] > 0.0 ? P.subDt[cell] : dtc;
         if (T0 < PAR(Q_TTHR)) {
             P.subDt[cell] = dtc;
+            if (P.cost) P.cost[cell] = 0;
             if (P.mode == 0) sfor<0, NS>([&](auto k) { P.R[k * P.ldR + cell] = 0.0; });
             return;  // stays PH_NEED
         }
@@ -306,7 +310,7 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
                 if (mine >= (unsigned long long)P.n) {
                     phase = PH_DONE;
                 } else {
-                    cell = (long long)mine;
+                    cell = P.order ? (long long)P.order[mine] : (long long)mine;
                     start_cell();
                 }
             }
@@ -352,6 +356,7 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
         if (phase == PH_NEWTON) {
             // NewtonIteration body after the solve (ImplicitEuler.cpp:200-249); delta in f
             CNT(S_NEWT) += 1;
+            CNT(S_CITER) += 1;
             bool stop = false;
             sfor<0, N>([&](auto i) {
                 if (!stop) {
@@ -404,7 +409,7 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
         }
         if (have) {
             attempt_done(aerr);
-        } else if (phase == PH_NEWTON && (CNT(S_CITER) += 1) > P.max_cell_iters) {
+        } else if (phase == PH_NEWTON && CNT(S_CITER) > P.max_cell_iters) {
             // no legitimate cell gets here; the cap guarantees every wavefront drains
             CNT(S_FAIL) += 1;
             finish_cell();

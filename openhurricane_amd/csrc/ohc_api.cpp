@@ -5,6 +5,7 @@
 // host<->device staging for host-pointer calls and kernel dispatch.  Every
 // entry point returns a status; nothing aborts.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -36,6 +37,12 @@ struct ohc_ctx {
     unsigned long long *d_ctl = nullptr;  // [0] queue head, [1..8] stats
     std::vector<void *> stage;            // host-pointer staging buffers
     std::vector<size_t> stage_sz;
+    // longest-first schedule: per-cell cost of the last call and its order
+    int64_t sched_n = 0;
+    bool sched_ready = false;
+    int *d_cost = nullptr, *d_order = nullptr, *d_iota = nullptr, *d_keys = nullptr;
+    void *d_sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
     std::string err, kname;
 };
 
@@ -141,6 +148,47 @@ int check_ctx(ohc_ctx *c) {
     return OHC_OK;
 }
 
+__global__ void iota_kernel(int *a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = (int)i;
+}
+
+void free_sched(ohc_ctx *c) {
+    for (void *p : {(void *)c->d_cost, (void *)c->d_order, (void *)c->d_iota, (void *)c->d_keys,
+                    c->d_sort_tmp})
+        if (p) (void)hipFree(p);
+    c->d_cost = c->d_order = c->d_iota = c->d_keys = nullptr;
+    c->d_sort_tmp = nullptr;
+    c->sort_tmp_bytes = 0;
+    c->sched_n = 0;
+    c->sched_ready = false;
+}
+
+// Buffers of the longest-first schedule for n cells (reallocated when n changes).
+int ensure_sched(ohc_ctx *c, int64_t n) {
+    if (c->sched_n == n && c->d_cost) return OHC_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_sched(c);
+    if (n > (int64_t)INT32_MAX) return fail(c, OHC_ERR_ARG, "schedule supports n < 2^31");
+    const size_t b = sizeof(int) * (size_t)n;
+    HIPCHK(c, hipMalloc((void **)&c->d_cost, b));
+    HIPCHK(c, hipMalloc((void **)&c->d_order, b));
+    HIPCHK(c, hipMalloc((void **)&c->d_iota, b));
+    HIPCHK(c, hipMalloc((void **)&c->d_keys, b));
+    size_t tmp = 0;
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, c->d_cost, c->d_keys,
+                                                           c->d_iota, c->d_order, (int)n, 0,
+                                                           32, c->stream));
+    HIPCHK(c, hipMalloc(&c->d_sort_tmp, tmp));
+    c->sort_tmp_bytes = tmp;
+    hipLaunchKernelGGL(iota_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       c->d_iota, n);
+    HIPCHK(c, hipGetLastError());
+    c->sched_n = n;
+    c->sched_ready = false;
+    return OHC_OK;
+}
+
 int read_stats(ohc_ctx *c, ohc_stats *st) {
     unsigned long long h[9];
     HIPCHK(c, hipMemcpyAsync(h, c->d_ctl, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -235,12 +283,25 @@ int chem_integrate(ohc_ctx *c, int mode, int64_t n, const double *T, double *Tio
     P.Tbeta = o.T_beta;
     P.dt_max = o.dt_max;
     P.max_cell_iters = 50000000;
+    if (o.schedule) {
+        if ((rc = ensure_sched(c, n))) return rc;
+        P.order = c->sched_ready ? c->d_order : nullptr;
+        P.cost = c->d_cost;
+    }
     P.counter = c->d_ctl;
     P.stats = c->d_ctl + 1;
     hipLaunchKernelGGL(reset_ctl, dim3(1), dim3(64), 0, c->stream, c->d_ctl);
     HIPCHK(c, hipGetLastError());
     const int grid = (int)std::min<int64_t>(c->grid, (n + 63) / 64);
     HIPCHK(c, c->lane->integrate(P, grid, c->stream));
+    if (o.schedule) {
+        // next call on these cells: most expensive first
+        size_t tmp = c->sort_tmp_bytes;
+        HIPCHK(c, hipcub::DeviceRadixSort::SortPairsDescending(c->d_sort_tmp, tmp, c->d_cost,
+                                                               c->d_keys, c->d_iota, c->d_order,
+                                                               (int)n, 0, 32, c->stream));
+        c->sched_ready = true;
+    }
     if ((rc = st.finish())) return rc;
     if (stats || !o.device_pointers) return read_stats(c, stats);
     return OHC_OK;
@@ -407,6 +468,7 @@ int ohc_ctx_free(ohc_ctx *c) {
     for (void *p : c->stage)
         if (p) (void)hipFree(p);
     if (c->d_ctl) (void)hipFree(c->d_ctl);
+    free_sched(c);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return OHC_OK;

@@ -144,3 +144,26 @@ def test_mix_props_and_T_from_h_match_oracle(cs, h2oracle, states, c1opts):
     np.testing.assert_allclose(Tg, To, rtol=1e-12)
     np.testing.assert_array_equal(fl, flo)
     np.testing.assert_allclose(Tg, st["T"], rtol=1e-4)
+
+
+def test_longest_first_schedule_is_bitwise_neutral(h2mech, c1opts, states):
+    """The cost-ordered schedule changes only which lane integrates a cell."""
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource
+
+    st, t = states
+    n = 2048
+    o1 = _lib.default_opts(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6, schedule=1)
+    cs1 = ChemistrySource(h2mech, opts=o1)
+    cs0 = ChemistrySource(h2mech, opts=c1opts)
+    dt = np.full(n, 1e-6)
+    outs = []
+    for cs_ in (cs0, cs1, cs1):  # index order, first scheduled call, reordered call
+        T, Y = st["T"].copy(), st["Y"].copy()
+        sub, s = cs_.solve_update(T, st["p"], st["rho"], Y, dt, 1.0)
+        outs.append((T, Y, sub, s))
+    for T, Y, sub, s in outs[1:]:
+        np.testing.assert_array_equal(T, outs[0][0])
+        np.testing.assert_array_equal(Y, outs[0][1])
+        np.testing.assert_array_equal(sub, outs[0][2])
+        assert s["newton_iters"] == outs[0][3]["newton_iters"]
