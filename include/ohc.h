@@ -112,8 +112,8 @@ typedef struct ohc_mech_tables {
     const int32_t *rev;           /* [nr] OHC_RXN_*                               */
     const int32_t *kind;          /* [nr] OHC_RATE_*                              */
     const double *arr;            /* [nr*3] A, beta, Ta (SI kmol-m^3-s-K): kf, or
-                                     k_inf for FALLOFF, k0 for CHEMACT              */
-    const double *arr_lo;         /* [nr*3] k0 for FALLOFF, k_inf for CHEMACT      */
+                                     k_inf for FALLOFF and CHEMACT                  */
+    const double *arr_lo;         /* [nr*3] k0 for FALLOFF and CHEMACT             */
     const double *arr_rev;        /* [nr*3] REV parameters (rev == REV_PARAMS)     */
     const int32_t *fo_type;       /* [nr] OHC_FO_*                                */
     const double *fo;             /* [nr*5] Troe alpha,T***,T*,T** / SRI a,b,c,d,e */
@@ -139,12 +139,22 @@ typedef struct ohc_opts {
     int32_t precision;      /* 0: FP64 everywhere; 1: FP32 rates/Gibbs/transport,
                                FP64 LU, T(h) Newton and sums                       */
     int32_t device_pointers;/* 1: array arguments are device pointers             */
-    int32_t kernel;         /* 0 auto, 1 lane-per-cell specialised, 2 generic     */
+    int32_t kernel;         /* 0 auto (specialised if compiled for this
+                               mechanism, else generic), 1 lane-per-cell
+                               specialised, 2 generic wavefront-per-cell (any
+                               mechanism with ns <= 63 and <= 3 distinct species
+                               per reaction side)                                  */
     int32_t waves_per_cu;   /* persistent-kernel occupancy target (0 = auto)      */
     int32_t schedule;       /* 0: cells in index order; 1: longest-first, ordered
                                by the per-cell Newton-iteration counts of the
                                previous call with the same n (first call: index
                                order).  Any order gives the same results.         */
+    int32_t jacobian;       /* 0: the reference's analytic Jacobian, including its
+                               sign convention for dKp/dT (reactionTypes.cpp:
+                               198-215); 1: exact dKp/dT -- same converged
+                               states within the Newton tolerance, far fewer
+                               Newton iterations when heat release couples
+                               strongly to the rates                              */
 } ohc_opts;
 
 /* Per-call counters (summed over cells) for roofline accounting. */

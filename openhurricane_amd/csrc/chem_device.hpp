@@ -332,7 +332,7 @@ struct MSumS {
 // reference's sign convention).
 template <class M, int r, bool WITH_D>
 __device__ __forceinline__ void equilibrium(const TFac &tf, const Thermo<M> &th, double &Kc,
-                                            double &rK, double &dKc) {
+                                            double &rK, double &dKc, bool exact) {
     constexpr RxnD d = M::R[r];
     constexpr bool zero_s = cabs(d.sumStoi) < kVeryTiny;
     double Kp, rKp = 0.0, dk = 0.0;
@@ -392,6 +392,7 @@ __device__ __forceinline__ void equilibrium(const TFac &tf, const Thermo<M> &th,
         rK = frcp(fmax(Kc, kVeryTiny));
     }
     if constexpr (WITH_D) {
+        if (exact) dk = -dk;  // exact d(Kp)/dT = -Kp d(sum nu G/(Ru T))/dT
         double dKp = Kp * dk;
         if (bad(dKp)) dKp = kVeryLarge;
         if constexpr (zero_s) dKc = dKp;
@@ -441,11 +442,12 @@ __device__ __forceinline__ FallOff<M, r> falloff(const TFac &tf, double Mtb) {
 
 // Evaluate dy/dt (and the Jacobian when WITH_J) at y = [c_1..c_ns, T] for the
 // constant-pressure reactor (FLOW == 0) or constant volume (FLOW == 1, pr =
-// rho0).  f has NS+1 entries, J is (NS+1)x(NS+1) row-major in registers.
+// rho0).  f has NS+1 entries, J is (NS+1)x(NS+1) row-major in registers;
+// exact selects the exact d(Kp)/dT over the reference's sign convention.
 template <class M, bool WITH_J, int FLOW>
 __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
                                         double (&f)[M::NS + 1],
-                                        double (&J)[M::NS + 1][M::NS + 1]) {
+                                        double (&J)[M::NS + 1][M::NS + 1], bool exact = false) {
     constexpr int NS = M::NS, N = NS + 1;
     const double T = y[NS];
     const TFac tf = tfac(T);
@@ -490,7 +492,7 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
             else kf = fo.k0 * (1.0 / (1.0 + fo.Pr)) * fo.F;
         }
         if constexpr (d.rev == 1) {
-            equilibrium<M, r, WITH_J>(tf, th, Kc, rKc, dKc);
+            equilibrium<M, r, WITH_J>(tf, th, Kc, rKc, dKc, exact);
             kr = Kc > kVeryTiny ? kf * rKc : 0.0;
         } else if constexpr (d.rev == 2) {
             kr = arrh<2, R>(tf.lnT, tf.invT);

@@ -48,6 +48,7 @@ struct LaneParams {
     int *cost;                    // per-cell Newton iterations of this call, or null
     unsigned long long *counter;  // work-queue head, zeroed before launch
     unsigned long long *stats;    // see ohc_stats: [0..5] counters, [6] dt_min bits, [7] failures
+    int exact_dkp;                // 1: exact d(Kp)/dT in the Jacobian
 };
 
 enum : int { PH_NEED = 0, PH_NEWTON = 1, PH_ERR0 = 2, PH_DONE = 3 };
@@ -327,7 +328,7 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
             double yk[N];
 #pragma unroll
             for (int i = 0; i < N; ++i) yk[i] = YK(i);
-            eval_fJ<M, true, FLOW>(yk, pr, f, J);
+            eval_fJ<M, true, FLOW>(yk, pr, f, J, P.exact_dkp != 0);
         }
 
         bool have = false;
@@ -451,6 +452,7 @@ struct EvalParams {
     double *J;
     int64_t ldJ;
     int flow;
+    int exact;  // 1: exact d(Kp)/dT
 };
 
 template <class M>
@@ -460,8 +462,8 @@ __global__ __launch_bounds__(64) void eval_lane_kernel(const EvalParams P) {
     if (cell >= P.n) return;
     double y[N], f[N], J[N][N];
     sfor<0, N>([&](auto i) { y[i] = P.y[i * P.ldy + cell]; });
-    if (P.flow == 0) eval_fJ<M, true, 0>(y, P.pr[cell], f, J);
-    else eval_fJ<M, true, 1>(y, P.pr[cell], f, J);
+    if (P.flow == 0) eval_fJ<M, true, 0>(y, P.pr[cell], f, J, P.exact != 0);
+    else eval_fJ<M, true, 1>(y, P.pr[cell], f, J, P.exact != 0);
     sfor<0, N>([&](auto i) { P.f[i * P.ldf + cell] = f[i]; });
     if (P.J)
         sfor<0, N>([&](auto i) {

@@ -97,7 +97,8 @@ Section section(const std::vector<std::string> &lines, const std::vector<const c
         std::string rest = trim(strip_comment(lines[i])).substr(tok[0].size());
         size_t j = i + 1;
         std::vector<std::string> body;
-        if (!trim(rest).empty()) body.push_back(rest);
+        // "THERMO ALL" is an option of the keyword, not an item
+        if (!trim(rest).empty() && upper(trim(rest)) != "ALL") body.push_back(rest);
         for (; j < lines.size(); ++j) body.push_back(lines[j]);
         for (auto &b : body) {
             std::string bu = upper(trim(strip_comment(b)));

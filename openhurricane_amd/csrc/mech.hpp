@@ -48,7 +48,10 @@ void parse_chemkin(const std::string &chem, const std::string *thermo,
                    const std::string *transport, Mechanism &m);
 void load_from_tables(const ohc_mech_tables &t, Mechanism &m);
 
-// Seeded synthetic element-balanced mechanism.
+// Seeded synthetic element-balanced mechanism: the CHEMKIN, NASA-7 and
+// transport texts, and the mechanism parsed from them.
+void synthetic_chemkin(int ns, int nr, uint64_t seed, double frac_rev, double frac_tb,
+                       std::string &chem, std::string &thermo, std::string &transport);
 void synthetic_mechanism(int ns, int nr, uint64_t seed, double frac_rev, double frac_tb,
                          Mechanism &m);
 

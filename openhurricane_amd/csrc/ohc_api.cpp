@@ -20,6 +20,8 @@
 #include "../../include/ohc.h"
 #include "lane_kernels.hpp"
 #include "mech.hpp"
+#include "wave_dispatch.hpp"
+#include "wave_kernels.hpp"
 
 struct ohc_mech {
     ohc::Mechanism m;
@@ -33,6 +35,9 @@ struct ohc_ctx {
     bool own_stream = false;
     const ohcd::LaneEntry *lane = nullptr;  // specialised kernels for this mechanism
     bool lane_transport = false;
+    bool wave = false;          // generic wavefront-per-cell kernels (run-time tables)
+    ohcd::WMech wm{};           // device tables of the generic kernels
+    void *d_wm = nullptr;       // one allocation holding them
     int grid = 0;  // persistent integrator grid (single-wavefront blocks)
     unsigned long long *d_ctl = nullptr;  // [0] queue head, [1..8] stats
     std::vector<void *> stage;            // host-pointer staging buffers
@@ -189,6 +194,130 @@ int ensure_sched(ohc_ctx *c, int64_t n) {
     return OHC_OK;
 }
 
+// Device tables of the generic kernels: species fields [field][64], reaction
+// fields [field][nrp], efficiencies [nr][64] and per-species reaction lists.
+int build_wmech(ohc_ctx *c) {
+    using namespace ohcd;
+    const ohc::Mechanism &m = c->mech;
+    const int ns = m.ns, nr = m.nr, P = OHC_MAXP;
+    if (ns + 1 > 64 || wave_size(ns + 1) < 0)
+        return fail(c, OHC_ERR_UNSUPPORTED, "generic kernels support ns <= 63");
+    for (int r = 0; r < nr; ++r)
+        if (m.nf[r] > kWP || m.nb[r] > kWP)
+            return fail(c, OHC_ERR_UNSUPPORTED,
+                        "generic kernels support at most 3 distinct species per reaction side");
+    const int nrp = std::max(64, (nr + 63) / 64 * 64);
+    std::vector<double> sp((size_t)SP_NFIELD * 64, 0.0);
+    for (int k = 0; k < 64; ++k) sp[SP_W * 64 + k] = 1.0;
+    for (int k = 0; k < ns; ++k) {
+        const double Ri = ohc::kRu / m.W[k];
+        sp[SP_W * 64 + k] = m.W[k];
+        sp[SP_TLOW * 64 + k] = m.Tlow[k];
+        sp[SP_THIGH * 64 + k] = m.Thigh[k];
+        sp[SP_TCOM * 64 + k] = m.Tcommon[k];
+        for (int side = 0; side < 2; ++side) {
+            const double *raw = side == 0 ? &m.nasa_lo[7 * k] : &m.nasa_hi[7 * k];
+            double a[7];
+            for (int i = 0; i < 7; ++i) a[i] = raw[i] * Ri;
+            const int oa = side == 0 ? SP_ALO : SP_AHI, oh = side == 0 ? SP_HLO : SP_HHI,
+                      os = side == 0 ? SP_SLO : SP_SHI, og = side == 0 ? SP_GLO : SP_GHI;
+            for (int i = 0; i < 7; ++i) sp[(oa + i) * 64 + k] = a[i];
+            const double hq[4] = {a[4] / 5.0, a[3] / 4.0, a[2] / 3.0, a[1] / 2.0};
+            const double sq[3] = {a[4] / 4.0, a[3] / 3.0, a[2] / 2.0};
+            const double gq[4] = {a[4] / 20.0, a[3] / 12.0, a[2] / 6.0, a[1] / 2.0};
+            for (int i = 0; i < 4; ++i) sp[(oh + i) * 64 + k] = hq[i];
+            for (int i = 0; i < 3; ++i) sp[(os + i) * 64 + k] = sq[i];
+            for (int i = 0; i < 4; ++i) sp[(og + i) * 64 + k] = gq[i];
+        }
+    }
+    std::vector<int> ri((size_t)RI_NFIELD * nrp, 0);
+    std::vector<double> rd((size_t)RD_NFIELD * nrp, 0.0), eff((size_t)std::max(nr, 1) * 64, 0.0);
+    std::vector<std::vector<std::pair<int, double>>> lists(ns);
+    for (int r = 0; r < nr; ++r) {
+        ri[RI_NF * nrp + r] = m.nf[r];
+        ri[RI_NB * nrp + r] = m.nb[r];
+        ri[RI_REV * nrp + r] = m.rev[r];
+        ri[RI_KIND * nrp + r] = m.kind[r];
+        ri[RI_FOT * nrp + r] = m.fo_type[r];
+        std::vector<double> net(ns, 0.0);
+        double s = 0.0;
+        for (int i = 0; i < m.nf[r]; ++i) {
+            const int k = m.f_idx[r * P + i];
+            ri[(RI_FI + i) * nrp + r] = k;
+            rd[(RD_FNU + i) * nrp + r] = m.f_nu[r * P + i];
+            rd[(RD_FO + i) * nrp + r] = m.f_ord[r * P + i];
+            net[k] -= m.f_nu[r * P + i];
+            s -= m.f_nu[r * P + i];
+        }
+        for (int i = 0; i < m.nb[r]; ++i) {
+            const int k = m.b_idx[r * P + i];
+            ri[(RI_BI + i) * nrp + r] = k;
+            rd[(RD_BNU + i) * nrp + r] = m.b_nu[r * P + i];
+            rd[(RD_BO + i) * nrp + r] = m.b_ord[r * P + i];
+            net[k] += m.b_nu[r * P + i];
+            s += m.b_nu[r * P + i];
+        }
+        for (int i = 0; i < 3; ++i) {
+            rd[(RD_A + i) * nrp + r] = m.arr[3 * r + i];
+            rd[(RD_A0 + i) * nrp + r] = m.arr_lo[3 * r + i];
+            rd[(RD_AR + i) * nrp + r] = m.arr_rev[3 * r + i];
+        }
+        for (int i = 0; i < 5; ++i) rd[(RD_FOP + i) * nrp + r] = m.fo[5 * r + i];
+        rd[RD_SUMS * nrp + r] = s;
+        if (m.kind[r] != OHC_RATE_ARRHENIUS)
+            for (int k = 0; k < ns; ++k) eff[(size_t)r * 64 + k] = m.eff[(size_t)r * ns + k];
+        for (int k = 0; k < ns; ++k)
+            if (net[k] != 0.0) lists[k].push_back({r, net[k]});
+    }
+    std::vector<int> cs_start(ns + 1, 0), cs_r;
+    std::vector<double> cs_nu;
+    for (int k = 0; k < ns; ++k) {
+        cs_start[k] = (int)cs_r.size();
+        for (auto &e : lists[k]) {
+            cs_r.push_back(e.first);
+            cs_nu.push_back(e.second);
+        }
+    }
+    cs_start[ns] = (int)cs_r.size();
+    if (cs_r.empty()) {
+        cs_r.push_back(0);
+        cs_nu.push_back(0.0);
+    }
+    // one device allocation, 256-byte aligned pieces
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t b_sp = al(sp.size() * 8), b_ri = al(ri.size() * 4), b_rd = al(rd.size() * 8),
+                 b_ef = al(eff.size() * 8), b_cs = al(cs_start.size() * 4), b_cr = al(cs_r.size() * 4),
+                 b_cn = al(cs_nu.size() * 8);
+    const size_t total = b_sp + b_ri + b_rd + b_ef + b_cs + b_cr + b_cn;
+    std::vector<unsigned char> h(total, 0);
+    size_t off = 0;
+    auto put = [&](const void *src, size_t bytes, size_t span) {
+        std::memcpy(h.data() + off, src, bytes);
+        const size_t o = off;
+        off += span;
+        return o;
+    };
+    const size_t o_sp = put(sp.data(), sp.size() * 8, b_sp), o_ri = put(ri.data(), ri.size() * 4, b_ri),
+                 o_rd = put(rd.data(), rd.size() * 8, b_rd), o_ef = put(eff.data(), eff.size() * 8, b_ef),
+                 o_cs = put(cs_start.data(), cs_start.size() * 4, b_cs),
+                 o_cr = put(cs_r.data(), cs_r.size() * 4, b_cr),
+                 o_cn = put(cs_nu.data(), cs_nu.size() * 8, b_cn);
+    HIPCHK(c, hipMalloc(&c->d_wm, total));
+    HIPCHK(c, hipMemcpy(c->d_wm, h.data(), total, hipMemcpyHostToDevice));
+    unsigned char *d = (unsigned char *)c->d_wm;
+    c->wm.ns = ns;
+    c->wm.nr = nr;
+    c->wm.nrp = nrp;
+    c->wm.sp = (const double *)(d + o_sp);
+    c->wm.ri = (const int *)(d + o_ri);
+    c->wm.rd = (const double *)(d + o_rd);
+    c->wm.eff = (const double *)(d + o_ef);
+    c->wm.cs_start = (const int *)(d + o_cs);
+    c->wm.cs_r = (const int *)(d + o_cr);
+    c->wm.cs_nu = (const double *)(d + o_cn);
+    return OHC_OK;
+}
+
 int read_stats(ohc_ctx *c, ohc_stats *st) {
     unsigned long long h[9];
     HIPCHK(c, hipMemcpyAsync(h, c->d_ctl, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -233,10 +362,8 @@ int chem_integrate(ohc_ctx *c, int mode, int64_t n, const double *T, double *Tio
         (mode == 0 && !R))
         return fail(c, OHC_ERR_ARG, "null array argument");
     if (!(dtFactor > 0)) return fail(c, OHC_ERR_ARG, "dtFactor must be > 0");
-    if (!c->lane)
-        return fail(c, OHC_ERR_UNSUPPORTED,
-                    "no device integrator compiled for this mechanism (hash mismatch); "
-                    "specialised kernels exist for: " + c->kname);
+    if (!c->lane && !c->wave)
+        return fail(c, OHC_ERR_UNSUPPORTED, "no device integrator for this mechanism");
     const size_t sv = sizeof(double) * (size_t)n;
     const size_t sY = sizeof(double) * (size_t)(ldY * (ns - 1) + n);
     Staging st(c);
@@ -290,10 +417,20 @@ int chem_integrate(ohc_ctx *c, int mode, int64_t n, const double *T, double *Tio
     }
     P.counter = c->d_ctl;
     P.stats = c->d_ctl + 1;
+    P.exact_dkp = o.jacobian;
     hipLaunchKernelGGL(reset_ctl, dim3(1), dim3(64), 0, c->stream, c->d_ctl);
     HIPCHK(c, hipGetLastError());
-    const int grid = (int)std::min<int64_t>(c->grid, (n + 63) / 64);
-    HIPCHK(c, c->lane->integrate(P, grid, c->stream));
+    if (c->lane) {
+        const int grid = (int)std::min<int64_t>(c->grid, (n + 63) / 64);
+        HIPCHK(c, c->lane->integrate(P, grid, c->stream));
+    } else {
+        ohcd::WaveParams W;
+        W.L = P;
+        W.M = c->wm;
+        W.exact_dkp = o.jacobian;
+        const int grid = (int)std::min<int64_t>(c->grid, n);
+        HIPCHK(c, ohcd::wave_integrate(W, grid, c->stream));
+    }
     if (o.schedule) {
         // next call on these cells: most expensive first
         size_t tmp = c->sort_tmp_bytes;
@@ -422,6 +559,8 @@ int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_
         !(o.T_beta > 0) || !(o.T_beta < 1))
         return fail(nullptr, OHC_ERR_ARG, "step-control factors out of range");
     if (o.precision != 0) return fail(nullptr, OHC_ERR_UNSUPPORTED, "precision mode not available");
+    if (o.jacobian != 0 && o.jacobian != 1) return fail(nullptr, OHC_ERR_ARG, "jacobian must be 0 or 1");
+    if (o.kernel < 0 || o.kernel > 2) return fail(nullptr, OHC_ERR_ARG, "kernel must be 0, 1 or 2");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(nullptr, OHC_ERR_DEVICE, "no HIP device available");
@@ -445,17 +584,26 @@ int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_
             c->lane_transport = th != 0 && ent[i].thash == th;
         }
     }
-    if (c->lane) {
+    if (!c->lane && o.kernel == 1)
+        return fail(nullptr, OHC_ERR_UNSUPPORTED,
+                    "no specialised kernel compiled for this mechanism; compiled: " +
+                        (names.empty() ? std::string("none") : names));
+    if (!c->lane) {
+        int rc = build_wmech(cp);
+        if (rc) return fail(nullptr, rc, cp->err);
+        c->wave = true;
+    }
+    {
         hipDeviceProp_t prop;
         HIPCHK(cp, hipGetDeviceProperties(&prop, device));
         int per_cu = 0;
-        HIPCHK(cp, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, c->lane->integrate_fn, 64, 0));
+        const void *fn = c->lane ? c->lane->integrate_fn : ohcd::wave_integrate_fn(c->mech.ns + 1);
+        HIPCHK(cp, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, 0));
         if (o.waves_per_cu > 0) per_cu = std::min(per_cu, o.waves_per_cu);
         if (per_cu < 1) per_cu = 1;
         c->grid = prop.multiProcessorCount * per_cu;
-        c->kname = std::string("lane:") + c->lane->name;
-    } else {
-        c->kname = names.empty() ? "none" : names;
+        c->kname = c->lane ? std::string("lane:") + c->lane->name
+                           : "wave:" + std::to_string(ohcd::wave_size(c->mech.ns + 1));
     }
     *out = c.release();
     return OHC_OK;
@@ -468,6 +616,7 @@ int ohc_ctx_free(ohc_ctx *c) {
     for (void *p : c->stage)
         if (p) (void)hipFree(p);
     if (c->d_ctl) (void)hipFree(c->d_ctl);
+    if (c->d_wm) (void)hipFree(c->d_wm);
     free_sched(c);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -527,7 +676,8 @@ static int eval_common(ohc_ctx *c, int64_t n, const double *y, int64_t ldy, cons
     if (n < 0 || ldy < n || ldf < n || (J && ldJ < n)) return fail(c, OHC_ERR_ARG, "bad sizes");
     if (!y || !p || !f) return fail(c, OHC_ERR_ARG, "null array argument");
     if (n == 0) return OHC_OK;
-    if (!c->lane) return fail(c, OHC_ERR_UNSUPPORTED, "no device kernels for this mechanism");
+    if (!c->lane && !c->wave) return fail(c, OHC_ERR_UNSUPPORTED, "no device kernels for this mechanism");
+    if (!c->lane && n > 0x7fffffffll) return fail(c, OHC_ERR_ARG, "n too large");
     Staging st(c);
     void *dy, *dp, *df, *dJ = nullptr;
     if ((rc = st.map(y, sizeof(double) * (size_t)(ldy * (N - 1) + n), true, false, &dy))) return rc;
@@ -535,9 +685,10 @@ static int eval_common(ohc_ctx *c, int64_t n, const double *y, int64_t ldy, cons
     if ((rc = st.map(f, sizeof(double) * (size_t)(ldf * (N - 1) + n), false, true, &df))) return rc;
     if (J && (rc = st.map(J, sizeof(double) * (size_t)(ldJ * (N * N - 1) + n), false, true, &dJ)))
         return rc;
-    ohcd::EvalParams P{n, (const double *)dy, ldy, (const double *)dp, (double *)df, ldf,
-                       (double *)dJ, ldJ, c->opts.flow_type};
-    HIPCHK(c, c->lane->eval(P, c->stream));
+    ohcd::EvalParams P{n,     (const double *)dy, ldy, (const double *)dp, (double *)df,
+                       ldf,   (double *)dJ,       ldJ, c->opts.flow_type,  c->opts.jacobian};
+    if (c->lane) HIPCHK(c, c->lane->eval(P, c->stream));
+    else HIPCHK(c, ohcd::wave_eval(P, c->wm, c->opts.jacobian, c->stream));
     return st.finish();
 }
 

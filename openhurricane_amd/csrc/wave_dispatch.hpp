@@ -1,0 +1,19 @@
+// wave_dispatch.hpp -- host interface of the generic wavefront-per-cell kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lane_kernels.hpp"
+
+namespace ohcd {
+
+struct WMech;
+struct WaveParams;
+
+// Padded ODE size the generic kernels use for n = ns + 1 unknowns (-1: too large).
+int wave_size(int n);
+hipError_t wave_integrate(const WaveParams &p, int grid, hipStream_t s);
+hipError_t wave_eval(const EvalParams &p, const WMech &m, int exact, hipStream_t s);
+const void *wave_integrate_fn(int n);
+
+}  // namespace ohcd

@@ -1,0 +1,818 @@
+// wave_kernels.hpp -- generic (run-time mechanism table) kernels: one
+// wavefront integrates one cell at a time.
+//
+// The lane kernels keep a whole cell in one lane's registers, which works up
+// to ~10 species.  Larger mechanisms (32 species / 150 reactions, 54 / 270)
+// need a (ns+1)^2 Jacobian per cell, so here the 64 lanes of a wavefront share
+// one cell:
+//   * species k lives in lane k: thermo, net rate omega_k, Jacobian row k;
+//   * reactions are evaluated 64 at a time, reaction r in lane r % 64, and
+//     their rates and partial derivatives are handed to the species lanes
+//     through LDS, where each species lane adds the contributions of its own
+//     reactions (a per-species reaction list) to its Jacobian row -- no
+//     atomics, and omega_k sums its reactions in reaction order;
+//   * the LU factorisation keeps row i of I - h J in lane i's registers and
+//     eliminates with partial pivoting across lanes (pivot search by wave
+//     reduction, pivot row broadcast lane to all), forward substitution
+//     folded into the elimination;
+//   * the ImplicitEuler control logic is uniform across the wavefront.
+// The arithmetic follows the same reference functions as chem_device.hpp and
+// lane_kernels.hpp (chemistrySource.cpp:863-1055, reactionTypes.cpp,
+// reactionRateTypes, ImplicitEuler.cpp:40-342, SquareMatrix.hpp:185-238).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "chem_device.hpp"
+#include "lane_kernels.hpp"
+
+namespace ohcd {
+
+constexpr int kWP = 3;  // distinct species per reaction side in the run-time tables
+
+// species fields, [field][64]
+enum {
+    SP_W,
+    SP_TLOW,
+    SP_THIGH,
+    SP_TCOM,
+    SP_ALO,
+    SP_AHI = SP_ALO + 7,
+    SP_HLO = SP_AHI + 7,
+    SP_HHI = SP_HLO + 4,
+    SP_SLO = SP_HHI + 4,
+    SP_SHI = SP_SLO + 3,
+    SP_GLO = SP_SHI + 3,
+    SP_GHI = SP_GLO + 4,
+    SP_NFIELD = SP_GHI + 4
+};
+// reaction int fields, [field][nrp]
+enum { RI_NF, RI_NB, RI_FI, RI_BI = RI_FI + kWP, RI_REV = RI_BI + kWP, RI_KIND, RI_FOT, RI_NFIELD };
+// reaction double fields, [field][nrp]
+enum {
+    RD_FNU,
+    RD_BNU = RD_FNU + kWP,
+    RD_FO = RD_BNU + kWP,
+    RD_BO = RD_FO + kWP,
+    RD_A = RD_BO + kWP,
+    RD_B,
+    RD_TA,
+    RD_A0,
+    RD_B0,
+    RD_TA0,
+    RD_AR,
+    RD_BR,
+    RD_TAR,
+    RD_FOP,
+    RD_SUMS = RD_FOP + 5,
+    RD_NFIELD
+};
+
+// Device copy of a mechanism (built by the host from the SoA tables).
+struct WMech {
+    int ns, nr, nrp;      // nrp = nr rounded up to a multiple of 64
+    const double *sp;     // [SP_NFIELD][64]
+    const int *ri;        // [RI_NFIELD][nrp]
+    const double *rd;     // [RD_NFIELD][nrp]
+    const double *eff;    // [nr][64] third-body efficiencies
+    const int *cs_start;  // [ns + 1] per-species reaction lists (CSR, by reaction)
+    const int *cs_r;      // reaction index
+    const double *cs_nu;  // net stoichiometric coefficient (products - reactants)
+};
+
+struct WaveParams {
+    LaneParams L;
+    WMech M;
+    int exact_dkp;  // 1: exact d(Kp)/dT instead of the reference's sign convention
+};
+
+__device__ __forceinline__ double rdlane(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wsum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ double wmax(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+// x^p for a run-time exponent with the integer fast paths of cpow<P>.
+__device__ __forceinline__ double rpow(double x, double p) {
+    if (p == 1.0) return x;
+    if (p == 2.0) return x * x;
+    if (p == 0.0) return 1.0;
+    if (p == 3.0) return x * x * x;
+    return pow(x, p);
+}
+__device__ __forceinline__ double xpow(const TFac &tf, double s) {
+    if (s == 1.0) return tf.X;
+    if (s == -1.0) return tf.Xinv;
+    if (s == 2.0) return tf.X * tf.X;
+    if (s == -2.0) return tf.Xinv * tf.Xinv;
+    return pow(tf.X, s);
+}
+// ArrheniusRR::k: A T^b exp(-Ta/T); with b = 0 or Ta = 0 the single exp gives
+// the same value as the reference's skipped pow / exp.
+__device__ __forceinline__ double rarrh(double A, double b, double Ta, const TFac &tf) {
+    return A * exp(b * tf.lnT - Ta * tf.invT);
+}
+__device__ __forceinline__ double rarrh_dkdT(double k, double b, double Ta, const TFac &tf) {
+    return k * tf.invTm * (b + Ta * tf.invTm);
+}
+
+// fallOffFunctions.cpp:54-168 with run-time type and parameters.
+__device__ double rfo_F(int fot, const double *fp, double T, double Pr) {
+    if (fot == 3) {
+        const double logPr = log10(fmax(Pr, kTiny));
+        const double X = 1.0 / (1.0 + logPr * logPr);
+        return fp[3] * pow(fp[0] * exp(-fp[1] / T) + exp(-T / fp[2]), X) * pow(T, fp[4]);
+    }
+    if (fot == 1 || fot == 2) {
+        double fc = (1.0 - fp[0]) * exp(-T / fp[1]) + fp[0] * exp(-T / fp[2]);
+        if (fot == 2) fc += exp(-fp[3] / T);
+        const double logFcent = log10(fmax(fc, kTiny));
+        const double c = -0.4 - 0.67 * logFcent, n = 0.75 - 1.27 * logFcent, dd = 0.14;
+        const double logPr = log10(fmax(Pr, kTiny));
+        const double q = (logPr + c) / (n - dd * (logPr + c));
+        return pow(10.0, logFcent / (1.0 + q * q));
+    }
+    return 1.0;
+}
+__device__ double rfo_DFDci(int fot, const double *fp, double T, double Pr, double F, double dPr) {
+    if (fot == 3) {
+        const double lp = log10(fmax(Pr, kTiny));
+        const double X = 1.0 / (1.0 + lp * lp);
+        const double dXdc = -X * X * 2.0 * log10(Pr) / Pr / log(10.0) * dPr;
+        return F * dXdc * log(fp[0] * exp(-fp[1] / T) + exp(-T / fp[2]));
+    }
+    if (fot == 1 || fot == 2) {
+        double fc = (1.0 - fp[0]) * exp(-T / fp[1]) + fp[0] * exp(-T / fp[2]);
+        if (fot == 2) fc += exp(-fp[3] / T);
+        const double logFcent = log10(fmax(fc, kTiny));
+        const double c = -0.4 - 0.67 * logFcent, n = 0.75 - 1.27 * logFcent, dd = 0.14;
+        const double logPrpc = log10(fmax(Pr, kTiny)) + c;
+        const double nmd = n - dd * logPrpc;
+        const double inv = logPrpc / nmd;
+        const double q = 1.0 + inv * inv;
+        return F * ((-logFcent) / (q * q)) * (2.0 * logPrpc / (nmd * nmd)) * (1.0 + dd * inv) *
+               (1.0 / Pr * dPr);
+    }
+    return 0.0;
+}
+__device__ double rfo_DFDT(int fot, const double *fp, double T, double Pr, double F, double dPrdT) {
+    if (fot == 3) {
+        const double a = fp[0], b = fp[1], c = fp[2], e = fp[4];
+        const double lp = log10(fmax(Pr, kTiny));
+        const double X = 1.0 / (1.0 + lp * lp);
+        const double dXdPr = -X * X * 2.0 * log10(Pr) / Pr / log(10.0);
+        const double ea = exp(-b / T), ec = exp(-T / c);
+        return F * (e / T + dXdPr * dPrdT * log(a * ea + ec) +
+                    X * (a * b * ea / (T * T) - ec / c) / (a * ea + ec));
+    }
+    if (fot == 1 || fot == 2) {
+        const double ln10 = log(10.0);
+        const double logPr = log10(fmax(Pr, kTiny));
+        const double e1 = exp(-T / fp[1]), e2 = exp(-T / fp[2]);
+        double Fc = (1.0 - fp[0]) * e1 + fp[0] * e2;
+        double dFc = (1.0 - fp[0]) * e1 / (-fp[1]) + fp[0] * e2 / (-fp[2]);
+        if (fot == 2) {
+            const double e3 = exp(-fp[3] / T);
+            Fc += e3;
+            dFc += e3 * (fp[3] / (T * T));
+        }
+        Fc = fmax(Fc, kTiny);
+        const double logFcent = log10(Fc);
+        const double dlF = 1.0 / (Fc * ln10) * dFc;
+        const double c = -0.4 - 0.67 * logFcent, dc = -0.67 * dlF;
+        const double n = 0.75 - 1.27 * logFcent, dn = -1.27 * dlF;
+        const double dd = 0.14;
+        const double dlogPr = dPrdT / Pr / ln10;
+        const double lpc = logPr + c;
+        const double nmd = n - dd * lpc;
+        const double inv = lpc / nmd;
+        const double q = 1.0 + inv * inv;
+        const double dPP = 2.0 * lpc / (nmd * nmd) *
+                           ((dlogPr + dc) - lpc / nmd * (dn - dd * (dlogPr + dc)));
+        return F * ln10 * (dlF / q - logFcent / (q * q) * dPP);
+    }
+    return 0.0;
+}
+
+// Molar G, dG/dT, Cp, H, dCp/dT of species k (JANAF.cpp:142-219), same
+// operation order as species_G / species_CH of the lane kernels.
+__device__ __forceinline__ void wthermo(const WMech &M, int k, const TFac &tf, double &G,
+                                        double &dG, double &Cp, double &H, double &dCp) {
+    const double *S = M.sp;
+    const double T = tf.T;
+    const bool lo = T < S[SP_TCOM * 64 + k];
+    const int oa = lo ? SP_ALO : SP_AHI, oh = lo ? SP_HLO : SP_HHI, os = lo ? SP_SLO : SP_SHI,
+              og = lo ? SP_GLO : SP_GHI;
+    double a[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) a[c] = S[(oa + c) * 64 + k];
+    const double h4 = S[(oh + 0) * 64 + k], h3 = S[(oh + 1) * 64 + k], h2 = S[(oh + 2) * 64 + k],
+                 h1 = S[(oh + 3) * 64 + k];
+    const double s4 = S[(os + 0) * 64 + k], s3 = S[(os + 1) * 64 + k], s2 = S[(os + 2) * 64 + k];
+    const double g4 = S[(og + 0) * 64 + k], g3 = S[(og + 1) * 64 + k], g2 = S[(og + 2) * 64 + k],
+                 g1 = S[(og + 3) * 64 + k];
+    const double W = S[SP_W * 64 + k], Tl = S[SP_TLOW * 64 + k], Th = S[SP_THIGH * 64 + k];
+    const bool out = (T < Tl) || (T > Th);
+    const double T0 = out ? fmin(fmax(T, Tl), Th) : T;
+    const double cp0 = ((((a[4] * T0 + a[3]) * T0 + a[2]) * T0 + a[1]) * T0 + a[0]);
+    const double s0 = ((((s4 * T + s3) * T + s2) * T + a[1]) * T + a[0] * tf.lnT + a[6]);
+    const double g0 = (((((-g4 * T - g3) * T - g2) * T - g1) * T + a[0] * (1.0 - tf.lnT) - a[6]) * T + a[5]);
+    const double ds0 = (a[0] * tf.invTm + a[1] + (a[2] + (a[3] + a[4] * T) * T) * T);
+    G = g0 * W;
+    dG = (cp0 - (s0 + T * ds0)) * W;
+    const double Tx = out ? (T < Tl ? Tl : Th) : T;
+    double h = ((((h4 * Tx + h3) * Tx + h2) * Tx + h1) * Tx + a[0]) * Tx + a[5];
+    if (out) h += cp0 * (T - Tx);
+    const double dcp = (a[1] + T * (2.0 * a[2] + T * (3.0 * a[3] + 4.0 * a[4] * T)));
+    Cp = cp0 * W;
+    H = h * W;
+    dCp = dcp * W;
+}
+
+// One reaction (lane-private): net rate q, and with J: dq/dT + dgamma/dT term
+// s, the third-body coefficient gam (Jacobian term gam * eff_j), and the
+// partial derivatives of q_f / q_r by each participant's concentration.
+struct WRxn {
+    double q, s, gam, dqf[kWP], dqr[kWP];
+};
+
+template <bool WITH_J>
+__device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf, const double *sc,
+                                          const double *sG, const double *sdG, int exact,
+                                          WRxn &o) {
+    const int nrp = M.nrp;
+    const int *RI = M.ri;
+    const double *RD = M.rd;
+    const int nf = RI[RI_NF * nrp + r], nb = RI[RI_NB * nrp + r];
+    const int rev = RI[RI_REV * nrp + r], kind = RI[RI_KIND * nrp + r];
+    int fi[kWP], bi[kWP];
+    double fnu[kWP], bnu[kWP], fo[kWP], bo[kWP], cf[kWP], cb[kWP];
+#pragma unroll
+    for (int i = 0; i < kWP; ++i) {
+        fi[i] = RI[(RI_FI + i) * nrp + r];
+        bi[i] = RI[(RI_BI + i) * nrp + r];
+        fnu[i] = RD[(RD_FNU + i) * nrp + r];
+        bnu[i] = RD[(RD_BNU + i) * nrp + r];
+        fo[i] = RD[(RD_FO + i) * nrp + r];
+        bo[i] = RD[(RD_BO + i) * nrp + r];
+        cf[i] = i < nf ? sc[fi[i]] : 1.0;
+        cb[i] = i < nb ? sc[bi[i]] : 1.0;
+    }
+    const double A = RD[RD_A * nrp + r], b = RD[RD_B * nrp + r], Ta = RD[RD_TA * nrp + r];
+    // third-body concentration, summed in species order (thirdBodyEfficiency::M)
+    double Mtb = 0.0;
+    if (kind != 0) {
+        const double *ef = M.eff + (size_t)r * 64;
+        for (int k = 0; k < M.ns; ++k) Mtb += ef[k] * sc[k];
+    }
+    double kf, k0 = 0.0, kinf = 0.0, Pr = 0.0, F = 1.0;
+    const double *fop = RD + RD_FOP * nrp + r;  // stride nrp
+    double fp[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) fp[i] = fop[i * nrp];
+    const int fot = RI[RI_FOT * nrp + r];
+    if (kind == 0) {
+        kf = rarrh(A, b, Ta, tf);
+    } else if (kind == 1) {
+        kf = Mtb * rarrh(A, b, Ta, tf);
+    } else {
+        k0 = rarrh(RD[RD_A0 * nrp + r], RD[RD_B0 * nrp + r], RD[RD_TA0 * nrp + r], tf);
+        kinf = rarrh(A, b, Ta, tf);
+        const double kinfz = kinf != 0 ? kinf : kVeryTiny;
+        Pr = k0 / kinfz * Mtb;
+        F = rfo_F(fot, fp, tf.T, Pr);
+        kf = kind == 2 ? kinf * (Pr / (1.0 + Pr)) * F : k0 * (1.0 / (1.0 + Pr)) * F;
+    }
+    double kr = 0.0, rK = 0.0, dKc = 0.0;
+    if (rev == 1) {
+        // reactionTypes::Kp / Kc / DKcDT (reactionTypes.cpp:178-241)
+        double Kt = 0.0, dk = 0.0;
+#pragma unroll
+        for (int i = 0; i < kWP; ++i) {
+            if (i < nf) {
+                const double g = sG[fi[i]];
+                Kt -= g * fnu[i];
+                if (WITH_J) dk -= (sdG[fi[i]] - g * tf.invTm) * fnu[i];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kWP; ++i) {
+            if (i < nb) {
+                const double g = sG[bi[i]];
+                Kt += g * bnu[i];
+                if (WITH_J) dk += (sdG[bi[i]] - g * tf.invTm) * bnu[i];
+            }
+        }
+        Kt *= tf.invRuT;
+        double Kp = exp(-Kt);
+        if (bad(Kp)) Kp = kRootVeryLarge;
+        const double sS = RD[RD_SUMS * nrp + r];
+        const bool zero_s = fabs(sS) < kVeryTiny;
+        double Xs = 1.0, Kc;
+        if (zero_s) {
+            Kc = Kp;
+        } else {
+            Xs = xpow(tf, sS);
+            Kc = fmin(Kp * Xs, kVeryLarge);
+        }
+        rK = frcp(fmax(Kc, kVeryTiny));
+        kr = Kc > kVeryTiny ? kf * rK : 0.0;
+        if (WITH_J) {
+            dk *= tf.invRuT;
+            if (exact) dk = -dk;
+            double dKp = Kp * dk;
+            if (bad(dKp)) dKp = kVeryLarge;
+            if (zero_s) {
+                dKc = dKp;
+            } else {
+                double v = (dKp + Kp * (-sS) * tf.invT) * Xs;
+                if (bad(v)) v = kVeryLarge;
+                dKc = v;
+            }
+        }
+    } else if (rev == 2) {
+        kr = rarrh(RD[RD_AR * nrp + r], RD[RD_BR * nrp + r], RD[RD_TAR * nrp + r], tf);
+        if (kind == 1) kr = Mtb * kr;
+    }
+    double qf = kf, qr = kr;
+#pragma unroll
+    for (int i = 0; i < kWP; ++i) {
+        if (i < nf) qf *= rpow(cf[i], fo[i]);
+        if (i < nb) qr *= rpow(cb[i], bo[i]);
+    }
+    const double q = qf - qr;
+    o.q = q;
+    if (!WITH_J) return;
+    // DqDci (reactionTypes.cpp:354-445)
+#pragma unroll
+    for (int i = 0; i < kWP; ++i) {
+        double df = kf, dr = kr;
+#pragma unroll
+        for (int j = 0; j < kWP; ++j) {
+            if (j < nf) {
+                if (j == i) {
+                    const double ord = fo[j], c = cf[j];
+                    if (ord == 1.0) {
+                    } else if (ord < 1.0) {
+                        df *= c > kTiny ? ord * pow(c, ord - 1.0) : 0.0;
+                    } else {
+                        df *= ord * rpow(c, ord - 1.0);
+                    }
+                } else {
+                    df *= rpow(cf[j], fo[j]);
+                }
+            }
+            if (j < nb) {
+                if (j == i) {
+                    const double ord = bo[j], c = cb[j];
+                    if (ord == 1.0) {
+                    } else if (ord < 1.0) {
+                        dr *= c > kTiny ? ord * pow(c, ord - 1.0) : 0.0;
+                    } else {
+                        dr *= ord * rpow(c, ord - 1.0);
+                    }
+                } else {
+                    dr *= rpow(cb[j], bo[j]);
+                }
+            }
+        }
+        o.dqf[i] = i < nf ? df : 0.0;
+        o.dqr[i] = i < nb ? dr : 0.0;
+    }
+    // third-body / fall-off terms q dln(gamma)/dc_j = gam * eff_j
+    double gam = 0.0;
+    if (kind == 1) {
+        gam = q * (1.0 / fmax(Mtb, kTiny));
+    } else if (kind >= 2 && Mtb > kTiny) {
+        const double kinfz = kinf != 0 ? kinf : kVeryTiny;
+        const double dPr = k0 / kinfz;
+        const double dF = rfo_DFDci(fot, fp, tf.T, Pr, F, dPr);
+        const double g = kind == 2 ? dPr / (Pr * (1 + Pr)) + dF / F : -dPr / (1 + Pr) + dF / F;
+        gam = bad(g) ? 0.0 : q * g;
+    }
+    o.gam = gam;
+    // DqDT (reactionTypes.cpp:843-878)
+    double dkf, dkr = 0.0;
+    if (kind <= 1) dkf = rarrh_dkdT(kf, b, Ta, tf);
+    else if (kind == 2) dkf = (Pr / (1.0 + Pr)) * F * rarrh_dkdT(kinf, b, Ta, tf);
+    else dkf = (1.0 / (1.0 + Pr)) * F * rarrh_dkdT(k0, RD[RD_B0 * nrp + r], RD[RD_TA0 * nrp + r], tf);
+    if (rev == 1) dkr = (dkf - kr * dKc) * rK;
+    else if (rev == 2) dkr = rarrh_dkdT(kr, RD[RD_BR * nrp + r], RD[RD_TAR * nrp + r], tf);
+#pragma unroll
+    for (int i = 0; i < kWP; ++i) {
+        if (i < nf) dkf *= rpow(cf[i], fo[i]);
+        if (i < nb) dkr *= rpow(cb[i], bo[i]);
+    }
+    double dqdT = dkf - dkr, dgdT = 0.0;
+    if (kind >= 2 && Mtb > kTiny) {
+        const double k0z = k0 != 0 ? k0 : kVeryTiny, kinfz = kinf != 0 ? kinf : kVeryTiny;
+        const double dPrdT =
+            Pr * (rarrh_dkdT(k0, RD[RD_B0 * nrp + r], RD[RD_TA0 * nrp + r], tf) / k0z -
+                  rarrh_dkdT(kinf, b, Ta, tf) / kinfz);
+        const double dFdT = rfo_DFDT(fot, fp, tf.T, Pr, F, dPrdT);
+        const double gg = kind == 2 ? dPrdT / (Pr * (1 + Pr)) + dFdT / F : -dPrdT / (1 + Pr) + dFdT / F;
+        dgdT = q * gg;
+    }
+    if (bad(dqdT)) dqdT = 0.0;
+    if (bad(dgdT)) dgdT = 0.0;
+    o.s = dqdT + dgdT;
+}
+
+// LDS work area of one wavefront.
+template <int N>
+struct WaveLDS {
+    double J[N * (N + 1)];  // Jacobian rows (stride N + 1)
+    double c[64], G[64], dG[64], Cp[64], H[64], dCp[64];
+    double q[64], s[64], gam[64], dqf[kWP][64], dqr[kWP][64];
+    double dinv[64];  // 1 / pivot of each elimination step
+    int perm[64];     // pivot lane of each elimination step
+};
+
+// dy/dt (and the Jacobian into L.J when WITH_J) at the wavefront-distributed
+// state yv (lane i holds y_i; lane ns holds T).  Returns f_i in lane i.
+template <int N, int FLOW, bool WITH_J>
+__device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, double yv, int exact) {
+    const int lane = threadIdx.x, ns = M.ns;
+    const double T = rdlane(yv, ns);
+    const TFac tf = tfac(T);
+    if (lane < ns) {
+        L.c[lane] = fmax(yv, 0.0);
+        double G, dG, Cp, H, dCp;
+        wthermo(M, lane, tf, G, dG, Cp, H, dCp);
+        L.G[lane] = G;
+        L.dG[lane] = dG;
+        L.Cp[lane] = Cp;
+        L.H[lane] = H;
+        L.dCp[lane] = dCp;
+    }
+    if (WITH_J)
+        for (int e = lane; e < N * (N + 1); e += 64) L.J[e] = 0.0;
+    __syncthreads();
+    double w = 0.0;
+    int ep = lane < ns ? M.cs_start[lane] : 0;
+    const int ee = lane < ns ? M.cs_start[lane + 1] : 0;
+    double *row = L.J + lane * (N + 1);
+    for (int base = 0; base < M.nr; base += 64) {
+        const int r = base + lane;
+        if (r < M.nr) {
+            WRxn o;
+            wreaction<WITH_J>(M, r, tf, L.c, L.G, L.dG, exact, o);
+            L.q[lane] = o.q;
+            if (WITH_J) {
+                L.s[lane] = o.s;
+                L.gam[lane] = o.gam;
+#pragma unroll
+                for (int i = 0; i < kWP; ++i) {
+                    L.dqf[i][lane] = o.dqf[i];
+                    L.dqr[i][lane] = o.dqr[i];
+                }
+            }
+        }
+        __syncthreads();
+        const int lim = base + 64;
+        while (ep < ee) {
+            const int rr = M.cs_r[ep];
+            if (rr >= lim) break;
+            const double nu = M.cs_nu[ep];
+            const int l = rr - base;
+            w += nu * L.q[l];
+            if (WITH_J) {
+                const int nrp = M.nrp;
+                const int nf = M.ri[RI_NF * nrp + rr], nb = M.ri[RI_NB * nrp + rr];
+#pragma unroll
+                for (int i = 0; i < kWP; ++i) {
+                    if (i < nf) row[M.ri[(RI_FI + i) * nrp + rr]] += nu * L.dqf[i][l];
+                    if (i < nb) row[M.ri[(RI_BI + i) * nrp + rr]] -= nu * L.dqr[i][l];
+                }
+                row[ns] += nu * L.s[l];
+                const double g = L.gam[l];
+                if (g != 0.0) {
+                    const double cg = nu * g;
+                    const double *ef = M.eff + (size_t)rr * 64;
+                    for (int j = 0; j < ns; ++j) row[j] += cg * ef[j];
+                }
+            }
+            ++ep;
+        }
+        __syncthreads();
+    }
+    // temperature equation (chemistrySource.cpp:863-907, jacobian :993-1054)
+    const bool sp = lane < ns;
+    double Hx = 0.0, Cx = 0.0;
+    if (sp) {
+        Hx = FLOW == 0 ? L.H[lane] : L.H[lane] - kRu * T;
+        Cx = FLOW == 0 ? L.Cp[lane] : L.Cp[lane] - kRu;
+    }
+    const double cC = wsum(sp ? yv * Cx : 0.0);
+    const double dTs = wsum(sp ? w * Hx : 0.0);
+    const double rcC = frcp(cC);
+    const double dT = dTs * rcC;
+    if (WITH_J) {
+        const double dcC = wsum(sp ? yv * L.dCp[lane] : 0.0);
+        double jt = 0.0;
+        if (sp) {
+            double v = 0.0;
+            for (int j = 0; j < ns; ++j) {
+                const double hx = FLOW == 0 ? L.H[j] : L.H[j] - kRu * T;
+                v -= L.J[j * (N + 1) + lane] * hx;
+            }
+            v += Cx * dT;
+            jt = v * rcC;
+        }
+        const double tt = wsum(sp ? (Cx * w + Hx * row[ns]) : 0.0);
+        __syncthreads();
+        if (sp) L.J[ns * (N + 1) + lane] = jt;
+        if (lane == 0) L.J[ns * (N + 1) + ns] = (dcC * dT - tt) * rcC;
+        __syncthreads();
+    }
+    return sp ? w : (lane == ns ? -dT : 0.0);
+}
+
+// Solve (I - h J) x = b with the Jacobian rows in LDS: lane i owns row i;
+// partial pivoting as SquareMatrix::LUDecomposeDPivoting (largest |pivot|),
+// forward substitution folded into the elimination.  Returns x_i in lane i.
+template <int N>
+__device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, double b) {
+    const int lane = threadIdx.x;
+    const bool act = lane < n;
+    const double *J = L.J;
+    double a[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const double v = act && j < n ? -h * J[lane * (N + 1) + j] : 0.0;
+        a[j] = (act && j == lane) ? v + 1.0 : v;
+    }
+    int step = act ? -1 : N;  // elimination step at which this row became the pivot
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        if (k < n) {
+            const double v = step < 0 ? fabs(a[k]) : -1.0;
+            const double m = wmax(v);
+            const unsigned long long hit = __ballot(step < 0 && v == m);
+            const int p = hit ? (int)__ffsll((long long)hit) - 1 : 0;  // no hit only for NaN rows
+            const double pk = rdlane(a[k], p);
+            const double rp = frcp(pk);
+            if (lane == 0) {
+                L.perm[k] = p;
+                L.dinv[k] = rp;
+            }
+            if (lane == p) step = k;
+            const bool el = step < 0;
+            const double l = el ? a[k] * rp : 0.0;
+            if (el) a[k] = l;
+#pragma unroll
+            for (int j = k + 1; j < N; ++j)
+                if (j < n) a[j] = fma(-l, rdlane(a[j], p), a[j]);
+            b = fma(-l, rdlane(b, p), b);
+        }
+    }
+    double x = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < N; ++kk) {
+        const int k = N - 1 - kk;
+        if (k < n) {
+            const double xk = rdlane(b, L.perm[k]) * L.dinv[k];
+            if (step < k) b = fma(-a[k], xk, b);
+            if (lane == k) x = xk;
+        }
+    }
+    return x;
+}
+
+// Persistent integrator: each wavefront takes one cell at a time from the
+// queue and runs chemistrySource::solve / solveUpdate on it (same control
+// flow as ie_lane_kernel, uniform across the wavefront).
+template <int N, int FLOW>
+__global__ __launch_bounds__(64) void ie_wave_kernel(const WaveParams WP) {
+    const LaneParams &P = WP.L;
+    const WMech &M = WP.M;
+    const int lane = threadIdx.x, ns = M.ns, n = ns + 1;
+    __shared__ WaveLDS<N> L;
+    __shared__ unsigned long long s_cell;
+    const bool sp = lane < ns;
+    const double W = sp ? M.sp[SP_W * 64 + lane] : 1.0;
+    const double errg = pow(P.maxsc / P.safe, -1.0 / P.alpha);
+    long long c_cells = 0, c_steps = 0, c_att = 0, c_newt = 0, c_rhs = 0, c_maxs = 0, c_fail = 0;
+    double dtmin = __builtin_huge_val();
+    for (;;) {
+        if (lane == 0) s_cell = atomicAdd(P.counter, 1ull);
+        __syncthreads();
+        const unsigned long long qi = s_cell;
+        __syncthreads();
+        if (qi >= (unsigned long long)P.n) break;
+        const long long cell = P.order ? (long long)P.order[qi] : (long long)qi;
+        const double T0 = P.T[cell];
+        const double dtc = P.dt[cell] * P.dtFactor;
+        const double sub_in = P.subDt[cell];
+        if (T0 < P.Tthr) {
+            if (lane == 0) {
+                P.subDt[cell] = dtc;
+                if (P.cost) P.cost[cell] = 0;
+            }
+            if (P.mode == 0 && sp) P.R[lane * P.ldR + cell] = 0.0;
+            continue;
+        }
+        const double rho = P.rho[cell];
+        const double pr = FLOW == 0 ? P.p[cell] : rho;
+        const double c0 = sp ? rho * P.Y[lane * P.ldY + cell] / W : 0.0;
+        double y = sp ? c0 : (lane == ns ? T0 : 0.0);  // chemistry state, lane i = y_i
+        double subDt = fmin(dtc, fmin(sub_in > 0.0 ? sub_in : dtc, P.dt_max));
+        double left = dtc;
+        long long csteps = 0, citer = 0;
+        bool failed = false;
+        // chemistrySource::solve: while (leftTime > 0) ODE solves of the remaining interval
+        while (left > 0.0 && !failed) {
+            double tn = left;
+            // ImplicitEuler::solve(t0, tn, dt0, y)
+            double t = 0.0, dt = 0.2 * subDt;
+            bool last = false, done = false;
+            double yl1 = y, lts0 = 0.0, lts1 = 0.0;
+            for (int nstep = 0; nstep < P.max_steps; ++nstep) {
+                const double y0 = y;
+                const double dtTry = dt;
+                if ((t + dt - tn) * (t + dt) > 0) {
+                    dt = tn - t;
+                    last = true;
+                } else if (t + dt == tn) {
+                    last = true;
+                }
+                // adaptiveSolve
+                double dtl = dt;
+                lts1 = lts0;
+                int chk = 0;
+                double yk, error;
+                for (;;) {
+                    lts0 = dtl;
+                    ++c_att;
+                    // ImplicitEuler::solve(t, dt, y0, y): Newton iterations
+                    yk = y0;
+                    int count = 0;
+                    double lastErr = 0.0, maxErr = 0.0;
+                    bool conv = true;
+                    for (;;) {
+                        const double f = wave_eval<N, FLOW, true>(M, L, yk, WP.exact_dkp);
+                        ++c_newt;
+                        ++citer;
+                        const double rhs = -yk + y0 + dtl * f;
+                        const double d = wave_lu_solve<N>(L, n, dtl, lane < n ? rhs : 0.0);
+                        if (__any(lane < n && bad(d))) {
+                            conv = false;
+                            maxErr = 1000.0;
+                            break;
+                        }
+                        yk += d;
+                        const double tol = P.atol + P.rtol * fabs(yk);
+                        maxErr = wmax(lane < n ? fabs(d) * frcp(tol) : 0.0);
+                        if (count > 0 && (maxErr / fmax(lastErr, kTiny)) > 2 && count > 5) {
+                            conv = false;
+                            maxErr = 1000.0;
+                            break;
+                        }
+                        lastErr = maxErr;
+                        ++count;
+                        if (isnan(maxErr)) {
+                            conv = false;
+                            maxErr = 1000.0;
+                            break;
+                        }
+                        if (count > 1000) {
+                            conv = false;
+                            break;
+                        }
+                        if (maxErr <= 1.0) break;
+                        if (citer > P.max_cell_iters) {
+                            conv = false;
+                            failed = true;
+                            break;
+                        }
+                    }
+                    if (failed) break;
+                    if (!conv) {
+                        error = maxErr;
+                    } else {
+                        double e;
+                        if (nstep == 0) {
+                            ++c_rhs;
+                            const double f = wave_eval<N, FLOW, false>(M, L, yk, WP.exact_dkp);
+                            e = yk - y0 - dtl * f;
+                        } else {
+                            const double dtn = lts0, dtn1 = lts1;
+                            const double f1 = 2.0 / (dtn * (dtn + dtn1));
+                            const double f2 = 2.0 / (dtn1 * (dtn + dtn1));
+                            const double LTEF = dtn * dtn * 0.5;
+                            e = (f1 * (yk - y0) - f2 * (y0 - yl1)) * LTEF;
+                        }
+                        const bool nan_y = __any(lane < n && isnan(yk));
+                        const double tol = P.atol_inc + P.rtol_inc * fmax(fabs(y0), fabs(yk));
+                        const double m = wmax(lane < n ? fabs(e) * frcp(tol) : 0.0);
+                        error = nan_y ? 1.5 : m;
+                    }
+                    // checkNewY, then the step-size control of adaptiveSolve
+                    const double y0T = rdlane(y0, ns), ykT = rdlane(yk, ns);
+                    const bool ok = !((ykT - y0T) > P.Talpha * y0T);
+                    if (!ok) dtl *= P.Tbeta;
+                    if (!ok && chk < 100) {
+                        ++chk;
+                        if (dtl <= kVeryTiny) {
+                            failed = true;
+                            break;
+                        }
+                        continue;
+                    }
+                    if (error > 1.0) {
+                        dtl *= fmax(P.safe * pow(error, -P.alpha), P.minsc);
+                        if (dtl <= kVeryTiny) {
+                            failed = true;
+                            break;
+                        }
+                        continue;
+                    }
+                    break;
+                }
+                if (failed) break;
+                // accepted step
+                t += dtl;
+                yl1 = y0;
+                y = yk;
+                if (error > errg)
+                    dt = fmin(fmax(P.safe * pow(error, -P.alpha), P.minsc), P.maxsc) * dtl;
+                else
+                    dt = fmax(P.safe * P.maxsc * dtl, dtl);
+                ++c_steps;
+                ++csteps;
+                if ((t - tn) * tn >= 0) {
+                    if (nstep > 0 && last) subDt = fmin(dtTry, tn);
+                    done = true;
+                    break;
+                }
+            }
+            if (failed) break;
+            if (!done) tn = t;  // step budget exhausted: time reached
+            left -= tn;
+        }
+        if (failed) ++c_fail;
+        // outputs (chemistrySource.cpp:1494-1609)
+        if (lane == 0) {
+            P.subDt[cell] = subDt;
+            if (P.cost) P.cost[cell] = (int)(citer < 0x7fffffffll ? citer : 0x7fffffffll);
+        }
+        if (P.mode == 0) {
+            if (sp) P.R[lane * P.ldR + cell] = (y - c0) * (1.0 / dtc) * W;
+        } else {
+            // ciToYi (chemistrySource.cpp:1222-1240)
+            const double yy = sp ? fmax(0.0, y) : 0.0;
+            const double rs = wsum(yy * W);
+            const double rr = 1.0 / fmax(rs, kTiny);
+            const double yv = yy * W * rr;
+            const double ys = wsum(yv);
+            const double ri = 1.0 / ys;
+            if (sp) P.Yout[lane * P.ldY + cell] = ys != 1.0 ? yv * ri : yv;
+            const double Tn = rdlane(y, ns);
+            if (lane == 0) P.Tout[cell] = Tn;
+        }
+        ++c_cells;
+        dtmin = fmin(dtmin, subDt);
+        c_maxs = csteps > c_maxs ? csteps : c_maxs;
+    }
+    if (lane == 0) {
+        atomicAdd(&P.stats[0], (unsigned long long)c_cells);
+        atomicAdd(&P.stats[1], (unsigned long long)c_steps);
+        atomicAdd(&P.stats[2], (unsigned long long)c_att);
+        atomicAdd(&P.stats[3], (unsigned long long)c_newt);
+        atomicAdd(&P.stats[4], (unsigned long long)c_rhs);
+        atomicMax(&P.stats[5], (unsigned long long)c_maxs);
+        if (dtmin < __builtin_huge_val())
+            atomicMin(&P.stats[6], (unsigned long long)__double_as_longlong(dtmin));
+        atomicAdd(&P.stats[7], (unsigned long long)c_fail);
+    }
+}
+
+// dy/dt and J at given states, one wavefront per cell.
+template <int N, int FLOW>
+__global__ __launch_bounds__(64) void eval_wave_kernel(const EvalParams P, const WMech M, int exact) {
+    __shared__ WaveLDS<N> L;
+    const int lane = threadIdx.x, ns = M.ns, n = ns + 1;
+    const int64_t cell = blockIdx.x;
+    if (cell >= P.n) return;
+    const double y = lane < n ? P.y[lane * P.ldy + cell] : 0.0;
+    const double pr = P.pr[cell];
+    (void)pr;
+    double f;
+    if (P.J) f = wave_eval<N, FLOW, true>(M, L, y, exact);
+    else f = wave_eval<N, FLOW, false>(M, L, y, exact);
+    if (lane < n) P.f[lane * P.ldf + cell] = f;
+    if (P.J && lane < n)
+        for (int j = 0; j < n; ++j) P.J[(lane * n + j) * P.ldJ + cell] = L.J[lane * (N + 1) + j];
+}
+
+}  // namespace ohcd

@@ -825,7 +825,7 @@ static double newton(ie_ws *s, const double *y0, double dt, double *ykp1, int *c
     int M[NMAX];
     double err[NMAX];
     do {
-        orc_jacobian(s->m, s->o->flow_type, s->pr, ykp1, s->f, s->J, 0);
+        orc_jacobian(s->m, s->o->flow_type, s->pr, ykp1, s->f, s->J, s->o->jacobian);
         s->newton++;
         for (int i = 0; i < n; ++i) {
             for (int j = 0; j < n; ++j) s->J[i * n + j] = -dt * s->J[i * n + j];
@@ -1259,7 +1259,7 @@ void orc_chem_source(const orc_mech *m, const ohc_opts *o, long n, const double 
                 diag_dqdci(m, r, kf, kr, q, pi, Ti, c, dg, calcLast);
                 /* DqDT into dwdT, reactionTypes.cpp:880-909 */
                 double dkfdT = rate_dkdT(m, r, m->arr + 3 * r, kf, Ti, c);
-                double dkrdT = rxn_dkrdT(m, r, kr, Ti, c, dkfdT, &tw, 0);
+                double dkrdT = rxn_dkrdT(m, r, kr, Ti, c, dkfdT, &tw, o->jacobian);
                 for (int i = 0; i < m->nf[r]; ++i)
                     dkfdT *= pow(dmax(c[m->fi[r * P + i]], 0.0), m->ford[r * P + i]);
                 for (int i = 0; i < m->nb[r]; ++i)

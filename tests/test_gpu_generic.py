@@ -1,0 +1,153 @@
+"""GPU parity of the generic wavefront-per-cell kernels (run-time mechanism
+tables) against the FP64 oracle, through the C-ABI.
+
+Cases: the H2-air mechanism forced onto the generic path (so the two device
+paths and the oracle agree on the same cells), and the synthetic 32-species /
+150-reaction mechanism (third-body, Lindemann / Troe / SRI fall-off), both
+reactor types and both Jacobian conventions.  Tolerances as for the lane
+kernels: RHS / Jacobian 1e-9 of the row scale; integrated states
+|dT|/T <= 1e-6, |dY| <= 1e-6 max(Y, 1e-3)."""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a HIP GPU")]
+
+
+def _opts(**kw):
+    from openhurricane_amd import _lib
+
+    base = dict(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6, kernel=2)
+    base.update(kw)
+    return _lib.default_opts(**base)
+
+
+@pytest.fixture(scope="module")
+def syn32():
+    from openhurricane_amd.chemistry import Mechanism
+
+    return Mechanism.synthetic(32, 150, seed=1)
+
+
+@pytest.fixture(scope="module")
+def syn32_oracle(syn32):
+    from oracle import Oracle
+
+    return Oracle(syn32)
+
+
+def _ode_states(st, W):
+    c = st["rho"] * st["Y"] / W[:, None]
+    return np.ascontiguousarray(np.vstack([c, st["T"][None, :]]))
+
+
+def _check_eval(cs, orc, y, p, flow, exact):
+    f, J = cs.eval_jacobian(y, p)
+    N = y.shape[0]
+    for i in range(y.shape[1]):
+        fo, Jo = orc.jacobian(y[:, i], p[i], flow=flow, exact_dkp=exact)
+        scale = np.max(np.abs(fo[:-1])) + 1e-300
+        assert np.all(np.abs(f[:-1, i] - fo[:-1]) <= 1e-9 * scale), i
+        assert abs(f[-1, i] - fo[-1]) <= 1e-9 * abs(fo[-1]) + 1e-9 * scale
+        row = np.max(np.abs(Jo), axis=1, keepdims=True) + 1e-300
+        err = np.abs(J[:, i].reshape(N, N) - Jo) / row
+        assert err.max() <= 1e-9, (i, err.max(), np.unravel_index(err.argmax(), err.shape))
+
+
+def _check_states(T, Y, To, Yo):
+    dT = np.abs(T - To) / To
+    dY = np.abs(Y - Yo) / np.maximum(np.abs(Yo), 1e-3)
+    print("max |dT|/T", dT.max(), "max dY", dY.max())
+    assert dT.max() <= 1e-6
+    assert dY.max() <= 1e-6
+
+
+def test_generic_h2air_matches_oracle(h2mech, h2oracle):
+    from openhurricane_amd.chemistry import ChemistrySource
+    from openhurricane_amd.sampler import h2air_states
+
+    o = _opts()
+    cs = ChemistrySource(h2mech, opts=o)
+    assert cs.kernel_name.startswith("wave:"), cs.kernel_name
+    t = h2mech.tables()
+    st = h2air_states(h2mech.species, t["W"], 512, seed=7)
+    y = _ode_states(st, t["W"])
+    _check_eval(cs, h2oracle, y[:, :128].copy(), np.ascontiguousarray(st["p"][:128]), 0, False)
+    dt = np.full(512, 1e-6)
+    To, Yo, sub_o, s_o = h2oracle.solve_update(o, st["T"], st["p"], st["rho"], st["Y"], dt, 1.0,
+                                               nthreads=16)
+    T, Y = st["T"].copy(), st["Y"].copy()
+    sub, s = cs.solve_update(T, st["p"], st["rho"], Y, dt, 1.0)
+    _check_states(T, Y, To, Yo)
+    assert s["cells"] == 512
+    assert abs(s["steps"] - s_o["steps"]) <= 0.01 * s_o["steps"]
+    np.testing.assert_allclose(sub, sub_o, rtol=1e-3)
+
+
+@pytest.mark.parametrize("flow,exact", [(0, 0), (0, 1), (1, 1)])
+def test_generic_synthetic_eval_matches_oracle(syn32, syn32_oracle, flow, exact):
+    from openhurricane_amd.chemistry import ChemistrySource
+    from openhurricane_amd.sampler import synthetic_workload
+
+    cs = ChemistrySource(syn32, opts=_opts(flow_type=flow, jacobian=exact))
+    t = syn32.tables()
+    st = synthetic_workload(syn32.species, t["W"], 96, seed=5)
+    y = _ode_states(st, t["W"])
+    # hot states too, so that the fall-off reactions run in their fall-off range
+    y[-1, ::3] = np.linspace(1700.0, 2600.0, y[:, ::3].shape[1])
+    pr = st["p"] if flow == 0 else st["rho"]
+    _check_eval(cs, syn32_oracle, y, np.ascontiguousarray(pr), flow, bool(exact))
+
+
+@pytest.mark.parametrize("flow,exact,n", [(0, 1, 96), (0, 0, 24), (1, 1, 48)])
+def test_generic_synthetic_solve_matches_oracle(syn32, syn32_oracle, flow, exact, n):
+    from openhurricane_amd.chemistry import ChemistrySource
+    from openhurricane_amd.sampler import synthetic_workload
+
+    o = _opts(flow_type=flow, jacobian=exact, atol_inc=1e-7, rtol_inc=1e-3)
+    cs = ChemistrySource(syn32, opts=o)
+    t = syn32.tables()
+    st = synthetic_workload(syn32.species, t["W"], n, seed=11 + flow)
+    dt = np.full(n, 1e-6)
+    To, Yo, sub_o, s_o = syn32_oracle.solve_update(o, st["T"], st["p"], st["rho"], st["Y"], dt,
+                                                   1.0, nthreads=16)
+    T, Y = st["T"].copy(), st["Y"].copy()
+    sub, s = cs.solve_update(T, st["p"], st["rho"], Y, dt, 1.0)
+    print("gpu", s, "oracle", s_o)
+    _check_states(T, Y, To, Yo)
+    assert abs(s["newton_iters"] - s_o["newton_iters"]) <= 0.02 * s_o["newton_iters"]
+    # element moles per unit mass drift only through the clipping of negative
+    # concentrations in ciToYi (loose step tolerances): the same drift as the oracle's
+    E, W = t["elem"], t["W"]
+    el0 = (st["Y"] / W[:, None]).T @ E
+    drift = ((Y / W[:, None]).T @ E - el0) / np.maximum(np.abs(el0), 1e-12)
+    drift_o = ((Yo / W[:, None]).T @ E - el0) / np.maximum(np.abs(el0), 1e-12)
+    np.testing.assert_allclose(drift, drift_o, rtol=0, atol=1e-9)
+    # source-term mode on the same cells
+    R, sub2, s2 = cs.solve(st["T"], st["p"], st["rho"], st["Y"], dt, 1.0)
+    Ro, _, _, _ = syn32_oracle.solve(o, st["T"], st["p"], st["rho"], st["Y"], dt, 1.0, nthreads=16)
+    scale = np.max(np.abs(Ro), axis=0) + 1e-300
+    assert np.max(np.abs(R - Ro) / scale) <= 1e-5
+
+
+def test_generic_schedule_is_bitwise_neutral(syn32):
+    from openhurricane_amd.chemistry import ChemistrySource
+    from openhurricane_amd.sampler import synthetic_workload
+
+    t = syn32.tables()
+    n = 256
+    st = synthetic_workload(syn32.species, t["W"], n, seed=4)
+    dt = np.full(n, 1e-6)
+    outs = []
+    cs1 = ChemistrySource(syn32, opts=_opts(jacobian=1, schedule=1, atol_inc=1e-7, rtol_inc=1e-3))
+    cs0 = ChemistrySource(syn32, opts=_opts(jacobian=1, schedule=0, atol_inc=1e-7, rtol_inc=1e-3))
+    for c in (cs0, cs1, cs1):
+        T, Y = st["T"].copy(), st["Y"].copy()
+        sub, s = c.solve_update(T, st["p"], st["rho"], Y, dt, 1.0)
+        outs.append((T, Y, sub))
+    for T, Y, sub in outs[1:]:
+        np.testing.assert_array_equal(T, outs[0][0])
+        np.testing.assert_array_equal(Y, outs[0][1])
+        np.testing.assert_array_equal(sub, outs[0][2])
