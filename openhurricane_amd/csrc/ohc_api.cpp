@@ -195,7 +195,7 @@ int ensure_sched(ohc_ctx *c, int64_t n) {
 }
 
 // Device tables of the generic kernels: species fields [field][64], reaction
-// fields [field][nrp], efficiencies [nr][64] and per-species reaction lists.
+// fields [field][nrp] and efficiencies [nr][64].
 int build_wmech(ohc_ctx *c) {
     using namespace ohcd;
     const ohc::Mechanism &m = c->mech;
@@ -232,21 +232,18 @@ int build_wmech(ohc_ctx *c) {
     }
     std::vector<int> ri((size_t)RI_NFIELD * nrp, 0);
     std::vector<double> rd((size_t)RD_NFIELD * nrp, 0.0), eff((size_t)std::max(nr, 1) * 64, 0.0);
-    std::vector<std::vector<std::pair<int, double>>> lists(ns);
     for (int r = 0; r < nr; ++r) {
         ri[RI_NF * nrp + r] = m.nf[r];
         ri[RI_NB * nrp + r] = m.nb[r];
         ri[RI_REV * nrp + r] = m.rev[r];
         ri[RI_KIND * nrp + r] = m.kind[r];
         ri[RI_FOT * nrp + r] = m.fo_type[r];
-        std::vector<double> net(ns, 0.0);
         double s = 0.0;
         for (int i = 0; i < m.nf[r]; ++i) {
             const int k = m.f_idx[r * P + i];
             ri[(RI_FI + i) * nrp + r] = k;
             rd[(RD_FNU + i) * nrp + r] = m.f_nu[r * P + i];
             rd[(RD_FO + i) * nrp + r] = m.f_ord[r * P + i];
-            net[k] -= m.f_nu[r * P + i];
             s -= m.f_nu[r * P + i];
         }
         for (int i = 0; i < m.nb[r]; ++i) {
@@ -254,7 +251,6 @@ int build_wmech(ohc_ctx *c) {
             ri[(RI_BI + i) * nrp + r] = k;
             rd[(RD_BNU + i) * nrp + r] = m.b_nu[r * P + i];
             rd[(RD_BO + i) * nrp + r] = m.b_ord[r * P + i];
-            net[k] += m.b_nu[r * P + i];
             s += m.b_nu[r * P + i];
         }
         for (int i = 0; i < 3; ++i) {
@@ -266,29 +262,12 @@ int build_wmech(ohc_ctx *c) {
         rd[RD_SUMS * nrp + r] = s;
         if (m.kind[r] != OHC_RATE_ARRHENIUS)
             for (int k = 0; k < ns; ++k) eff[(size_t)r * 64 + k] = m.eff[(size_t)r * ns + k];
-        for (int k = 0; k < ns; ++k)
-            if (net[k] != 0.0) lists[k].push_back({r, net[k]});
-    }
-    std::vector<int> cs_start(ns + 1, 0), cs_r;
-    std::vector<double> cs_nu;
-    for (int k = 0; k < ns; ++k) {
-        cs_start[k] = (int)cs_r.size();
-        for (auto &e : lists[k]) {
-            cs_r.push_back(e.first);
-            cs_nu.push_back(e.second);
-        }
-    }
-    cs_start[ns] = (int)cs_r.size();
-    if (cs_r.empty()) {
-        cs_r.push_back(0);
-        cs_nu.push_back(0.0);
     }
     // one device allocation, 256-byte aligned pieces
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t b_sp = al(sp.size() * 8), b_ri = al(ri.size() * 4), b_rd = al(rd.size() * 8),
-                 b_ef = al(eff.size() * 8), b_cs = al(cs_start.size() * 4), b_cr = al(cs_r.size() * 4),
-                 b_cn = al(cs_nu.size() * 8);
-    const size_t total = b_sp + b_ri + b_rd + b_ef + b_cs + b_cr + b_cn;
+                 b_ef = al(eff.size() * 8);
+    const size_t total = b_sp + b_ri + b_rd + b_ef;
     std::vector<unsigned char> h(total, 0);
     size_t off = 0;
     auto put = [&](const void *src, size_t bytes, size_t span) {
@@ -298,10 +277,7 @@ int build_wmech(ohc_ctx *c) {
         return o;
     };
     const size_t o_sp = put(sp.data(), sp.size() * 8, b_sp), o_ri = put(ri.data(), ri.size() * 4, b_ri),
-                 o_rd = put(rd.data(), rd.size() * 8, b_rd), o_ef = put(eff.data(), eff.size() * 8, b_ef),
-                 o_cs = put(cs_start.data(), cs_start.size() * 4, b_cs),
-                 o_cr = put(cs_r.data(), cs_r.size() * 4, b_cr),
-                 o_cn = put(cs_nu.data(), cs_nu.size() * 8, b_cn);
+                 o_rd = put(rd.data(), rd.size() * 8, b_rd), o_ef = put(eff.data(), eff.size() * 8, b_ef);
     HIPCHK(c, hipMalloc(&c->d_wm, total));
     HIPCHK(c, hipMemcpy(c->d_wm, h.data(), total, hipMemcpyHostToDevice));
     unsigned char *d = (unsigned char *)c->d_wm;
@@ -312,9 +288,6 @@ int build_wmech(ohc_ctx *c) {
     c->wm.ri = (const int *)(d + o_ri);
     c->wm.rd = (const double *)(d + o_rd);
     c->wm.eff = (const double *)(d + o_ef);
-    c->wm.cs_start = (const int *)(d + o_cs);
-    c->wm.cs_r = (const int *)(d + o_cr);
-    c->wm.cs_nu = (const double *)(d + o_cn);
     return OHC_OK;
 }
 

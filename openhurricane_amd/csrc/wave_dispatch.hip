@@ -1,28 +1,27 @@
-// wave_dispatch.hip -- instantiations of the generic (run-time table) kernels
-// for padded ODE sizes N and their launch helpers.
+// wave_dispatch.hip -- dispatch of the generic (run-time table) kernels over
+// the padded ODE sizes N.  Each size is instantiated in its own translation
+// unit (wave_size.hip compiled with -DOHC_WAVE_N=N), so they build in parallel.
 #include "wave_dispatch.hpp"
 
 #include "wave_kernels.hpp"
 
 namespace ohcd {
 
-namespace {
-template <int N>
-struct WaveN {
-    static hipError_t integrate(const WaveParams &p, int grid, hipStream_t s) {
-        if (p.L.flow == 0) hipLaunchKernelGGL((ie_wave_kernel<N, 0>), dim3(grid), dim3(64), 0, s, p);
-        else hipLaunchKernelGGL((ie_wave_kernel<N, 1>), dim3(grid), dim3(64), 0, s, p);
-        return hipGetLastError();
-    }
-    static hipError_t eval(const EvalParams &p, const WMech &m, int exact, hipStream_t s) {
-        const dim3 g((unsigned)p.n);
-        if (p.flow == 0) hipLaunchKernelGGL((eval_wave_kernel<N, 0>), g, dim3(64), 0, s, p, m, exact);
-        else hipLaunchKernelGGL((eval_wave_kernel<N, 1>), g, dim3(64), 0, s, p, m, exact);
-        return hipGetLastError();
-    }
-    static const void *fn() { return (const void *)&ie_wave_kernel<N, 0>; }
-};
+#define OHC_WAVE_DECL(n)                                                           \
+    hipError_t wave_integrate_##n(const WaveParams &, int, hipStream_t);             \
+    hipError_t wave_eval_##n(const EvalParams &, const WMech &, int, hipStream_t);  \
+    const void *wave_fn_##n();
+OHC_WAVE_DECL(12)
+OHC_WAVE_DECL(16)
+OHC_WAVE_DECL(24)
+OHC_WAVE_DECL(33)
+OHC_WAVE_DECL(40)
+OHC_WAVE_DECL(48)
+OHC_WAVE_DECL(55)
+OHC_WAVE_DECL(64)
+#undef OHC_WAVE_DECL
 
+namespace {
 struct Entry {
     int N;
     hipError_t (*integrate)(const WaveParams &, int, hipStream_t);
@@ -30,7 +29,7 @@ struct Entry {
     const void *(*fn)();
 };
 
-#define OHC_WAVE(n) Entry{n, &WaveN<n>::integrate, &WaveN<n>::eval, &WaveN<n>::fn}
+#define OHC_WAVE(n) Entry{n, &wave_integrate_##n, &wave_eval_##n, &wave_fn_##n}
 const Entry kWave[] = {OHC_WAVE(12), OHC_WAVE(16), OHC_WAVE(24), OHC_WAVE(33),
                        OHC_WAVE(40), OHC_WAVE(48), OHC_WAVE(55), OHC_WAVE(64)};
 #undef OHC_WAVE

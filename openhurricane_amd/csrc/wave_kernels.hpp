@@ -7,10 +7,9 @@
 // one cell:
 //   * species k lives in lane k: thermo, net rate omega_k, Jacobian row k;
 //   * reactions are evaluated 64 at a time, reaction r in lane r % 64, and
-//     their rates and partial derivatives are handed to the species lanes
-//     through LDS, where each species lane adds the contributions of its own
-//     reactions (a per-species reaction list) to its Jacobian row -- no
-//     atomics, and omega_k sums its reactions in reaction order;
+//     each reaction lane adds its contributions to omega and to the Jacobian
+//     rows of its participants with LDS floating-point atomics (ds_add_f64),
+//     straight from the registers that hold the reaction's data;
 //   * the LU factorisation keeps row i of I - h J in lane i's registers and
 //     eliminates with partial pivoting across lanes (pivot search by wave
 //     reduction, pivot row broadcast lane to all), forward substitution
@@ -77,9 +76,6 @@ struct WMech {
     const int *ri;        // [RI_NFIELD][nrp]
     const double *rd;     // [RD_NFIELD][nrp]
     const double *eff;    // [nr][64] third-body efficiencies
-    const int *cs_start;  // [ns + 1] per-species reaction lists (CSR, by reaction)
-    const int *cs_r;      // reaction index
-    const double *cs_nu;  // net stoichiometric coefficient (products - reactants)
 };
 
 struct WaveParams {
@@ -93,13 +89,39 @@ __device__ __forceinline__ double rdlane(double v, int l) {
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
     return __hiloint2double(hi, lo);
 }
+// Data-parallel-primitive move of a double within each row of 16 lanes
+// (quad permutations and row mirrors; no LDS traffic).
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141;  // row_half_mirror
+constexpr int kDppMirror = 0x140;      // row_mirror
+// Wavefront reductions: pairwise within rows by DPP, then the four row values
+// through scalar reads; every lane gets the same bits (each combine is
+// symmetric in its two operands).
 __device__ __forceinline__ double wsum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    v += dpp<kDppXor1>(v);
+    v += dpp<kDppXor2>(v);
+    v += dpp<kDppHalfMirror>(v);
+    v += dpp<kDppMirror>(v);
+    return (rdlane(v, 0) + rdlane(v, 16)) + (rdlane(v, 32) + rdlane(v, 48));
 }
 __device__ __forceinline__ double wmax(double v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return v;
+    v = fmax(v, dpp<kDppXor1>(v));
+    v = fmax(v, dpp<kDppXor2>(v));
+    v = fmax(v, dpp<kDppHalfMirror>(v));
+    v = fmax(v, dpp<kDppMirror>(v));
+    return fmax(fmax(rdlane(v, 0), rdlane(v, 16)), fmax(rdlane(v, 32), rdlane(v, 48)));
+}
+
+// LDS floating-point add (ds_add_f64), result unused.
+__device__ __forceinline__ void lds_add(double *p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // x^p for a run-time exponent with the integer fast paths of cpow<P>.
@@ -244,6 +266,8 @@ __device__ __forceinline__ void wthermo(const WMech &M, int k, const TFac &tf, d
 // partial derivatives of q_f / q_r by each participant's concentration.
 struct WRxn {
     double q, s, gam, dqf[kWP], dqr[kWP];
+    int nf, nb, fi[kWP], bi[kWP];
+    double fnu[kWP], bnu[kWP];
 };
 
 template <bool WITH_J>
@@ -273,6 +297,7 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
     double Mtb = 0.0;
     if (kind != 0) {
         const double *ef = M.eff + (size_t)r * 64;
+#pragma unroll 8
         for (int k = 0; k < M.ns; ++k) Mtb += ef[k] * sc[k];
     }
     double kf, k0 = 0.0, kinf = 0.0, Pr = 0.0, F = 1.0;
@@ -352,6 +377,15 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
     }
     const double q = qf - qr;
     o.q = q;
+    o.nf = nf;
+    o.nb = nb;
+#pragma unroll
+    for (int i = 0; i < kWP; ++i) {
+        o.fi[i] = fi[i];
+        o.bi[i] = bi[i];
+        o.fnu[i] = fnu[i];
+        o.bnu[i] = bnu[i];
+    }
     if (!WITH_J) return;
     // DqDci (reactionTypes.cpp:354-445)
 #pragma unroll
@@ -433,7 +467,7 @@ template <int N>
 struct WaveLDS {
     double J[N * (N + 1)];  // Jacobian rows (stride N + 1)
     double c[64], G[64], dG[64], Cp[64], H[64], dCp[64];
-    double q[64], s[64], gam[64], dqf[kWP][64], dqr[kWP][64];
+    double w[64];     // net production rates omega_k
     double dinv[64];  // 1 / pivot of each elimination step
     int perm[64];     // pivot lane of each elimination step
 };
@@ -457,55 +491,45 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
     }
     if (WITH_J)
         for (int e = lane; e < N * (N + 1); e += 64) L.J[e] = 0.0;
+    L.w[lane] = 0.0;
     __syncthreads();
-    double w = 0.0;
-    int ep = lane < ns ? M.cs_start[lane] : 0;
-    const int ee = lane < ns ? M.cs_start[lane + 1] : 0;
-    double *row = L.J + lane * (N + 1);
     for (int base = 0; base < M.nr; base += 64) {
         const int r = base + lane;
         if (r < M.nr) {
             WRxn o;
             wreaction<WITH_J>(M, r, tf, L.c, L.G, L.dG, exact, o);
-            L.q[lane] = o.q;
-            if (WITH_J) {
-                L.s[lane] = o.s;
-                L.gam[lane] = o.gam;
+            // d(omega)/dt rows of the participants: sigma_a q, and with J the
+            // row terms sigma_a dq/dc_j, sigma_a dq/dT and sigma_a gam eff_j
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
 #pragma unroll
                 for (int i = 0; i < kWP; ++i) {
-                    L.dqf[i][lane] = o.dqf[i];
-                    L.dqr[i][lane] = o.dqr[i];
+                    const bool has = side == 0 ? i < o.nf : i < o.nb;
+                    if (!has) continue;
+                    const int a = side == 0 ? o.fi[i] : o.bi[i];
+                    const double nu = side == 0 ? -o.fnu[i] : o.bnu[i];
+                    lds_add(&L.w[a], nu * o.q);
+                    if (WITH_J) {
+                        double *row = L.J + a * (N + 1);
+#pragma unroll
+                        for (int j = 0; j < kWP; ++j) {
+                            if (j < o.nf) lds_add(&row[o.fi[j]], nu * o.dqf[j]);
+                            if (j < o.nb) lds_add(&row[o.bi[j]], -nu * o.dqr[j]);
+                        }
+                        lds_add(&row[ns], nu * o.s);
+                        if (o.gam != 0.0) {
+                            const double cg = nu * o.gam;
+                            const double *ef = M.eff + (size_t)r * 64;
+#pragma unroll 8
+                            for (int j = 0; j < ns; ++j) lds_add(&row[j], cg * ef[j]);
+                        }
+                    }
                 }
             }
         }
-        __syncthreads();
-        const int lim = base + 64;
-        while (ep < ee) {
-            const int rr = M.cs_r[ep];
-            if (rr >= lim) break;
-            const double nu = M.cs_nu[ep];
-            const int l = rr - base;
-            w += nu * L.q[l];
-            if (WITH_J) {
-                const int nrp = M.nrp;
-                const int nf = M.ri[RI_NF * nrp + rr], nb = M.ri[RI_NB * nrp + rr];
-#pragma unroll
-                for (int i = 0; i < kWP; ++i) {
-                    if (i < nf) row[M.ri[(RI_FI + i) * nrp + rr]] += nu * L.dqf[i][l];
-                    if (i < nb) row[M.ri[(RI_BI + i) * nrp + rr]] -= nu * L.dqr[i][l];
-                }
-                row[ns] += nu * L.s[l];
-                const double g = L.gam[l];
-                if (g != 0.0) {
-                    const double cg = nu * g;
-                    const double *ef = M.eff + (size_t)rr * 64;
-                    for (int j = 0; j < ns; ++j) row[j] += cg * ef[j];
-                }
-            }
-            ++ep;
-        }
-        __syncthreads();
     }
+    __syncthreads();
+    const double w = lane < ns ? L.w[lane] : 0.0;
     // temperature equation (chemistrySource.cpp:863-907, jacobian :993-1054)
     const bool sp = lane < ns;
     double Hx = 0.0, Cx = 0.0;
@@ -522,6 +546,7 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
         double jt = 0.0;
         if (sp) {
             double v = 0.0;
+#pragma unroll 8
             for (int j = 0; j < ns; ++j) {
                 const double hx = FLOW == 0 ? L.H[j] : L.H[j] - kRu * T;
                 v -= L.J[j * (N + 1) + lane] * hx;
@@ -529,7 +554,8 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
             v += Cx * dT;
             jt = v * rcC;
         }
-        const double tt = wsum(sp ? (Cx * w + Hx * row[ns]) : 0.0);
+        const double jT = sp ? L.J[lane * (N + 1) + ns] : 0.0;  // d(omega_k)/dT
+        const double tt = wsum(sp ? (Cx * w + Hx * jT) : 0.0);
         __syncthreads();
         if (sp) L.J[ns * (N + 1) + lane] = jt;
         if (lane == 0) L.J[ns * (N + 1) + ns] = (dcC * dT - tt) * rcC;
