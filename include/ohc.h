@@ -136,8 +136,12 @@ typedef struct ohc_opts {
     double T_beta;          /* solveOptions/TemperatureBeta (0.25)                */
     double dt_max;          /* chemicalMaxStep (1e15)                             */
     double T_low, T_high;   /* thermoList TLow / THigh for T(h) limits (50, 5000) */
-    int32_t precision;      /* 0: FP64 everywhere; 1: FP32 rates/Gibbs/transport,
-                               FP64 LU, T(h) Newton and sums                       */
+    int32_t precision;      /* 0: FP64 everywhere; 1: split -- Gibbs energies,
+                               equilibrium and Arrhenius rate constants, species
+                               transport and mixing rules in FP32; concentration
+                               products, net rates, Jacobian, LU, T(h) Newton,
+                               mole fractions and mass sums in FP64 (compiled
+                               mechanisms only)                                    */
     int32_t device_pointers;/* 1: array arguments are device pointers             */
     int32_t kernel;         /* 0 auto (specialised if compiled for this
                                mechanism, else generic), 1 lane-per-cell
@@ -242,6 +246,61 @@ int ohc_thermo_T_from_h(ohc_ctx *ctx, int64_t n, const double *h, const double *
 int ohc_mix_props(ohc_ctx *ctx, int64_t n, const double *T, const double *p,
                   const double *Y, int64_t ldY, double *mu, double *kappa, double *Dk,
                   int64_t ldD, double *cp, double *ha, double *hk);
+
+/* ---- chemistry surrogate (no counterpart in the reference) ------------------
+ * A multilayer perceptron that maps one cell's state to its chemistry
+ * increment over the flow time step it was trained for:
+ *   inputs  [T, ln p, ln max(Y_k, 1e-20)]          (ns + 2)
+ *   outputs [T1 - T0, ln Y1_k - ln Y0_k]            (ns + 1)
+ * with `depth` hidden layers of `width` GELU units, BF16 weights and
+ * activations, FP32 accumulation on the matrix cores.  It is trained on state
+ * pairs produced by ohc_chem_solve_update itself.  The surrogate belongs to a
+ * context created with device_pointers = 1; every array argument below is
+ * device memory. */
+typedef struct ohc_surrogate ohc_surrogate;
+
+int ohc_surrogate_create(ohc_ctx *ctx, int32_t width, int32_t depth, uint64_t seed,
+                         ohc_surrogate **out);
+int ohc_surrogate_free(ohc_surrogate *s);
+int ohc_surrogate_num_params(const ohc_surrogate *s, int64_t *n);
+/* FP32 master parameters (host arrays): per layer W [out][in] then b [out] */
+int ohc_surrogate_get_params(ohc_surrogate *s, float *params);
+int ohc_surrogate_set_params(ohc_surrogate *s, const float *params);
+/* Host arrays: feature mean/std and the accepted feature box [lo, hi] (NULL:
+ * unbounded), target mean/std, per-output loss weights (NULL: 1), and the
+ * flow time step the pairs were generated with. */
+int ohc_surrogate_set_norm(ohc_surrogate *s, const float *in_mean, const float *in_std,
+                           const float *in_lo, const float *in_hi, const float *out_mean,
+                           const float *out_std, const float *out_weight, double dt_train);
+/* Raw features X [n][ns+2] of states (T0, p, Y0) and, when T1/Y1 are given,
+ * raw targets Yt [n][ns+1] of the pairs (T0, Y0) -> (T1, Y1). */
+int ohc_surrogate_features(ohc_surrogate *s, int64_t n, const double *T0, const double *p,
+                           const double *Y0, int64_t ldY, const double *T1, const double *Y1,
+                           float *X, float *Yt);
+/* One Adam step (lr) on a batch of raw features / targets; the batch loss
+ * (weighted mean-square error of the normalised targets) into *loss (host,
+ * nullable -- a non-null loss synchronises the stream). */
+int ohc_surrogate_train_step(ohc_surrogate *s, int64_t n, const float *X, const float *Yt,
+                             float lr, float *loss);
+/* Loss gradient with respect to every parameter (host array, parameter
+ * layout) for a batch -- the backward kernels without the update. */
+int ohc_surrogate_gradient(ohc_surrogate *s, int64_t n, const float *X, const float *Yt,
+                           float *grad, float *loss);
+/* Predicted raw increments dout [n][ns+1] at states (T, p, Y). */
+int ohc_surrogate_predict(ohc_surrogate *s, int64_t n, const double *T, const double *p,
+                          const double *Y, int64_t ldY, float *dout);
+/* solveUpdate with the surrogate: every cell is advanced by the network unless
+ * its gate fails -- time step differs from the training step, a feature lies
+ * outside the accepted box, the new T leaves [T_low, T_high], or the relative
+ * change of any element's mass exceeds gate_tol -- and the failed cells
+ * (count in *n_fallback) are integrated by the plain integrator, whose
+ * counters go to stats. */
+int ohc_chem_solve_update_surrogate(ohc_ctx *ctx, ohc_surrogate *s, int64_t n, double *T,
+                                    const double *p, const double *rho, double *Y, int64_t ldY,
+                                    const double *dt, double dtFactor, double *subDt,
+                                    double gate_tol, int64_t *n_fallback, ohc_stats *stats);
+int ohc_surrogate_save(const ohc_surrogate *s, const char *path);
+int ohc_surrogate_load(ohc_surrogate *s, const char *path);
 
 #ifdef __cplusplus
 }

@@ -93,6 +93,25 @@ _SIGS = [
                                 _V, _V, _V]),
 ]
 
+_FP = C.POINTER(C.c_float)
+_SIGS += [
+    ("ohc_surrogate_create", C.c_int, [_V, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(_V)]),
+    ("ohc_surrogate_free", C.c_int, [_V]),
+    ("ohc_surrogate_num_params", C.c_int, [_V, C.POINTER(C.c_int64)]),
+    ("ohc_surrogate_get_params", C.c_int, [_V, _V]),
+    ("ohc_surrogate_set_params", C.c_int, [_V, _V]),
+    ("ohc_surrogate_set_norm", C.c_int, [_V, _V, _V, _V, _V, _V, _V, _V, C.c_double]),
+    ("ohc_surrogate_features", C.c_int, [_V, C.c_int64, _V, _V, _V, C.c_int64, _V, _V, _V, _V]),
+    ("ohc_surrogate_train_step", C.c_int, [_V, C.c_int64, _V, _V, C.c_float, _FP]),
+    ("ohc_surrogate_gradient", C.c_int, [_V, C.c_int64, _V, _V, _V, _FP]),
+    ("ohc_surrogate_predict", C.c_int, [_V, C.c_int64, _V, _V, _V, C.c_int64, _V]),
+    ("ohc_chem_solve_update_surrogate", C.c_int, [_V, _V, C.c_int64, _V, _V, _V, _V, C.c_int64,
+                                                  _V, C.c_double, _V, C.c_double,
+                                                  C.POINTER(C.c_int64), C.POINTER(Stats)]),
+    ("ohc_surrogate_save", C.c_int, [_V, C.c_char_p]),
+    ("ohc_surrogate_load", C.c_int, [_V, C.c_char_p]),
+]
+
 EXPORTS = [s[0] for s in _SIGS]
 
 _lib = None

@@ -258,6 +258,19 @@ class ChemistrySource:
         return (T, flag) if want_flag else T
 
     # transportList::muKappa + Df, thermoList cp / ha
+    # chemistrySource::calculateSourceTerms(Imp): R [ns, n] and, with diag, the
+    # (coupled) diagonal d(omega_k)/d(c_k) [ns or ns-1, n]
+    def source_terms(self, T, p, rho, Y, diag=False, without_last=False):
+        n = T.shape[0]
+        R = self._zeros((self.ns, n))
+        D = self._zeros((self.ns, n)) if diag else None
+        rc = lib().ohc_chem_source(self._h, n, self._p(T), self._p(p), self._p(rho), self._p(Y),
+                                   _ld(Y, n), self._p(R), _ld(R, n),
+                                   self._p(D) if diag else None, _ld(D, n) if diag else n,
+                                   int(without_last))
+        check(rc, self._h)
+        return (R, D) if diag else R
+
     def mix_props(self, T, p, Y, mu=True, kappa=True, Dk=True, cp=True, ha=True, hk=False):
         n = T.shape[0]
         out = {}

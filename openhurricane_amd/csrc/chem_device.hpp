@@ -49,6 +49,17 @@ struct RxnD {
 };
 
 constexpr double cabs(double x) { return x < 0 ? -x : x; }
+// compile-time square root (Newton iterations to convergence) for constants
+constexpr double csqrt(double x) {
+    if (!(x > 0)) return 0.0;
+    double r = x > 1 ? x : 1.0;
+    for (int i = 0; i < 200; ++i) {
+        const double n = 0.5 * (r + x / r);
+        if (n == r) break;
+        r = n;
+    }
+    return r;
+}
 
 template <int B, int E, class F>
 __device__ __forceinline__ void sfor(F &&f) {
@@ -156,23 +167,59 @@ __device__ __forceinline__ void species_CH(double T, double &Cp, double &H, doub
     dCp = dcp * W;
 }
 
+// species_G evaluated in FP32 (the split-precision mode's Gibbs terms).
+template <class M, int k>
+__device__ __forceinline__ void species_G32(double Td, float lnT, float invTm, double &G,
+                                            double &dG) {
+    const bool lo = Td < M::Tcommon[k];
+    const float T = (float)Td;
+    float a[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) a[c] = lo ? (float)M::alo[k][c] : (float)M::ahi[k][c];
+    const float s4 = lo ? (float)M::slo[k][0] : (float)M::shi[k][0];
+    const float s3 = lo ? (float)M::slo[k][1] : (float)M::shi[k][1];
+    const float s2 = lo ? (float)M::slo[k][2] : (float)M::shi[k][2];
+    const float g4 = lo ? (float)M::glo[k][0] : (float)M::ghi[k][0];
+    const float g3 = lo ? (float)M::glo[k][1] : (float)M::ghi[k][1];
+    const float g2 = lo ? (float)M::glo[k][2] : (float)M::ghi[k][2];
+    const float g1 = lo ? (float)M::glo[k][3] : (float)M::ghi[k][3];
+    const float Tl = (float)M::Tlow[k], Th = (float)M::Thigh[k];
+    const float T0 = (T < Tl || T > Th) ? fminf(fmaxf(T, Tl), Th) : T;
+    const float cp0 = ((((a[4] * T0 + a[3]) * T0 + a[2]) * T0 + a[1]) * T0 + a[0]);
+    const float s0 = ((((s4 * T + s3) * T + s2) * T + a[1]) * T + a[0] * lnT + a[6]);
+    const float g0 = (((((-g4 * T - g3) * T - g2) * T - g1) * T + a[0] * (1.0f - lnT) - a[6]) * T + a[5]);
+    const float ds0 = (a[0] * invTm + a[1] + (a[2] + (a[3] + a[4] * T) * T) * T);
+    constexpr float W = (float)M::W[k];
+    G = (double)(g0 * W);
+    dG = (double)((cp0 - (s0 + T * ds0)) * W);
+}
+
 template <class M, int k>
 __device__ __forceinline__ void species_thermo(double T, double lnT, double invTm, Thermo<M> &th) {
     species_G<M, k>(T, lnT, invTm, th.G[k], th.dG[k]);
     species_CH<M, k>(T, th.Cp[k], th.H[k], th.dCp[k]);
 }
 
-// ArrheniusRR::k with the reference's skip of pow/exp when b or Ta vanish.
-template <int kindsel, class R>
+// ArrheniusRR::k with the reference's skip of pow/exp when b or Ta vanish;
+// F = float evaluates the rate constant in FP32 (split-precision mode).
+template <int kindsel, class R, class F = double>
 __device__ __forceinline__ double arrh(double lnT, double invT) {
     constexpr double A = kindsel == 0 ? R::A : (kindsel == 1 ? R::A0 : R::Ar);
     constexpr double b = kindsel == 0 ? R::b : (kindsel == 1 ? R::b0 : R::br);
     constexpr double Ta = kindsel == 0 ? R::Ta : (kindsel == 1 ? R::Ta0 : R::Tar);
     constexpr bool hb = cabs(b) > kVeryTiny, ht = cabs(Ta) > kVeryTiny;
-    if constexpr (hb && ht) return A * exp(b * lnT - Ta * invT);
-    else if constexpr (hb) return A * exp(b * lnT);
-    else if constexpr (ht) return A * exp(-Ta * invT);
-    else return A;
+    if constexpr (std::is_same_v<F, float>) {
+        const float l = (float)lnT, it = (float)invT;
+        if constexpr (hb && ht) return (double)((float)A * __expf((float)b * l - (float)Ta * it));
+        else if constexpr (hb) return (double)((float)A * __expf((float)b * l));
+        else if constexpr (ht) return (double)((float)A * __expf(-(float)Ta * it));
+        else return A;
+    } else {
+        if constexpr (hb && ht) return A * exp(b * lnT - Ta * invT);
+        else if constexpr (hb) return A * exp(b * lnT);
+        else if constexpr (ht) return A * exp(-Ta * invT);
+        else return A;
+    }
 }
 template <int kindsel, class R>
 __device__ __forceinline__ double arrh_dkdT(double kj, double invTm) {
@@ -330,13 +377,31 @@ struct MSumS {
 // Equilibrium terms of reversible reaction r: Kc, rK = 1/max(Kc, veryTiny)
 // and (optionally) dKc/dT -- reactionTypes.cpp:178-241 (dKp/dT with the
 // reference's sign convention).
-template <class M, int r, bool WITH_D>
+template <class M, int r, bool WITH_D, class F = double>
 __device__ __forceinline__ void equilibrium(const TFac &tf, const Thermo<M> &th, double &Kc,
                                             double &rK, double &dKc, bool exact) {
     constexpr RxnD d = M::R[r];
     constexpr bool zero_s = cabs(d.sumStoi) < kVeryTiny;
+    constexpr bool kc_species = M::KC_SPECIES && std::is_same_v<F, double>;
     double Kp, rKp = 0.0, dk = 0.0;
-    if constexpr (M::KC_SPECIES) {
+    if constexpr (std::is_same_v<F, float>) {
+        // FP32 Gibbs sums and exponential (split-precision mode)
+        float Kt = 0.0f, dkf = 0.0f;
+        const float iRuT = (float)tf.invRuT, iTm = (float)tf.invTm;
+        sfor<0, d.nf>([&](auto i) {
+            constexpr int k = d.fi[i];
+            Kt -= (float)th.G[k] * (float)d.fnu[i];
+            if constexpr (WITH_D) dkf -= ((float)th.dG[k] - (float)th.G[k] * iTm) * (float)d.fnu[i];
+        });
+        sfor<0, d.nb>([&](auto i) {
+            constexpr int k = d.bi[i];
+            Kt += (float)th.G[k] * (float)d.bnu[i];
+            if constexpr (WITH_D) dkf += ((float)th.dG[k] - (float)th.G[k] * iTm) * (float)d.bnu[i];
+        });
+        Kp = (double)__expf(-Kt * iRuT);
+        dk = (double)(dkf * iRuT);
+        if (bad(Kp)) Kp = kRootVeryLarge;
+    } else if constexpr (kc_species) {
         // th.G = E_k = exp(-G_k/(Ru T)), th.H = 1/E_k, th.dG = (dG_k - G_k/T)/(Ru T):
         // Kp = prod_products E^nu prod_reactants (1/E)^nu == exp(-sum nu G/(Ru T)).
         double Pn = 1.0, Qd = 1.0, Pi = 1.0, Qi = 1.0;
@@ -384,7 +449,7 @@ __device__ __forceinline__ void equilibrium(const TFac &tf, const Thermo<M> &th,
         Xs = Xpow<SumS<M, r>>(tf);
         Kc = fmin(Kp * Xs, kVeryLarge);
     }
-    if constexpr (M::KC_SPECIES) {
+    if constexpr (kc_species) {
         double rKc = rKp;
         if constexpr (!zero_s) rKc = Kc == kVeryLarge ? 1.0 / kVeryLarge : rKp * Xpow<MSumS<M, r>>(tf);
         rK = Kc > kVeryTiny ? rKc : 1.0 / kVeryTiny;
@@ -444,17 +509,22 @@ __device__ __forceinline__ FallOff<M, r> falloff(const TFac &tf, double Mtb) {
 // constant-pressure reactor (FLOW == 0) or constant volume (FLOW == 1, pr =
 // rho0).  f has NS+1 entries, J is (NS+1)x(NS+1) row-major in registers;
 // exact selects the exact d(Kp)/dT over the reference's sign convention.
-template <class M, bool WITH_J, int FLOW>
+// F = float is the split-precision mode: Gibbs energies, equilibrium and
+// Arrhenius rate constants in FP32, everything else (concentration products,
+// net rates, Jacobian, temperature equation) in FP64.
+template <class M, bool WITH_J, int FLOW, class F = double>
 __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
                                         double (&f)[M::NS + 1],
                                         double (&J)[M::NS + 1][M::NS + 1], bool exact = false) {
     constexpr int NS = M::NS, N = NS + 1;
+    constexpr bool fp32 = std::is_same_v<F, float>;
     const double T = y[NS];
     const TFac tf = tfac(T);
     Thermo<M> th;  // G / dG now; Cp / H / dCp after the reaction loop
     sfor<0, NS>([&](auto k) {
-        species_G<M, k>(T, tf.lnT, tf.invTm, th.G[k], th.dG[k]);
-        if constexpr (M::KC_SPECIES) {
+        if constexpr (fp32) species_G32<M, k>(T, (float)tf.lnT, (float)tf.invTm, th.G[k], th.dG[k]);
+        else species_G<M, k>(T, tf.lnT, tf.invTm, th.G[k], th.dG[k]);
+        if constexpr (M::KC_SPECIES && !fp32) {
             const double gRT = th.G[k] * tf.invRuT;
             th.dG[k] = (th.dG[k] - th.G[k] * tf.invTm) * tf.invRuT;
             th.G[k] = exp(-gRT);
@@ -484,18 +554,18 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
         if constexpr (d.kind != 0) Mtb = Mg[d.tb];
         double kf, kr = 0.0, Kc = 0.0, dKc = 0.0, rKc = 0.0;
         FallOff<M, r> fo{};
-        if constexpr (d.kind == 0) kf = arrh<0, R>(tf.lnT, tf.invT);
-        else if constexpr (d.kind == 1) kf = Mtb * arrh<0, R>(tf.lnT, tf.invT);
+        if constexpr (d.kind == 0) kf = arrh<0, R, F>(tf.lnT, tf.invT);
+        else if constexpr (d.kind == 1) kf = Mtb * arrh<0, R, F>(tf.lnT, tf.invT);
         else {
             fo = falloff<M, r>(tf, Mtb);
             if constexpr (d.kind == 2) kf = fo.kinf * (fo.Pr / (1.0 + fo.Pr)) * fo.F;
             else kf = fo.k0 * (1.0 / (1.0 + fo.Pr)) * fo.F;
         }
         if constexpr (d.rev == 1) {
-            equilibrium<M, r, WITH_J>(tf, th, Kc, rKc, dKc, exact);
+            equilibrium<M, r, WITH_J, F>(tf, th, Kc, rKc, dKc, exact);
             kr = Kc > kVeryTiny ? kf * rKc : 0.0;
         } else if constexpr (d.rev == 2) {
-            kr = arrh<2, R>(tf.lnT, tf.invT);
+            kr = arrh<2, R, F>(tf.lnT, tf.invT);
             if constexpr (d.kind == 1) kr = Mtb * kr;
         }
         // ---- concentration products

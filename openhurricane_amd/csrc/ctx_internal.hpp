@@ -1,0 +1,130 @@
+// ctx_internal.hpp -- definitions shared by the translation units behind the
+// C-ABI (mechanism and context objects, error reporting, host staging, the
+// integrator driver).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ohc.h"
+#include "lane_kernels.hpp"
+#include "mech.hpp"
+#include "wave_kernels.hpp"
+
+struct ohc_mech {
+    ohc::Mechanism m;
+};
+
+struct ohc_ctx {
+    ohc::Mechanism mech;
+    ohc_opts opts;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    const ohcd::LaneEntry *lane = nullptr;  // specialised kernels for this mechanism
+    bool lane_transport = false;
+    bool wave = false;          // generic wavefront-per-cell kernels (run-time tables)
+    ohcd::WMech wm{};           // device tables of the generic kernels
+    void *d_wm = nullptr;       // one allocation holding them
+    int grid = 0;  // persistent integrator grid (single-wavefront blocks)
+    unsigned long long *d_ctl = nullptr;  // [0] queue head, [1..8] stats
+    std::vector<void *> stage;            // host-pointer staging buffers
+    std::vector<size_t> stage_sz;
+    // longest-first schedule: per-cell cost of the last call and its order
+    int64_t sched_n = 0;
+    bool sched_ready = false;
+    int *d_cost = nullptr, *d_order = nullptr, *d_iota = nullptr, *d_keys = nullptr;
+    void *d_sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    std::string err, kname;
+};
+
+namespace ohci {
+
+int fail(ohc_ctx *c, int code, const std::string &msg);
+
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return ohci::fail(ctx, OHC_ERR_DEVICE,                                          \
+                              std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+// Device staging for host pointers: slot i holds at least bytes.
+inline int stage_buf(ohc_ctx *c, size_t slot, size_t bytes, void **out) {
+    if (c->stage.size() <= slot) {
+        c->stage.resize(slot + 1, nullptr);
+        c->stage_sz.resize(slot + 1, 0);
+    }
+    if (c->stage_sz[slot] < bytes) {
+        if (c->stage[slot]) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipFree(c->stage[slot]));
+        }
+        c->stage[slot] = nullptr;
+        c->stage_sz[slot] = 0;
+        HIPCHK(c, hipMalloc(&c->stage[slot], bytes));
+        c->stage_sz[slot] = bytes;
+    }
+    *out = c->stage[slot];
+    return OHC_OK;
+}
+
+// Bring one array to the device (host mode) or pass it through (device mode).
+struct Arr {
+    const void *host = nullptr;
+    void *dev = nullptr;
+    size_t bytes = 0;
+    bool out = false;
+};
+
+class Staging {
+   public:
+    explicit Staging(ohc_ctx *c) : c_(c) {}
+    // in: copy host->device; out: copy device->host at finish()
+    int map(const void *p, size_t bytes, bool in, bool out, void **dev) {
+        if (!p) {
+            *dev = nullptr;
+            return OHC_OK;
+        }
+        if (c_->opts.device_pointers) {
+            *dev = const_cast<void *>(p);
+            return OHC_OK;
+        }
+        void *d = nullptr;
+        int rc = stage_buf(c_, slot_++, bytes, &d);
+        if (rc) return rc;
+        if (in) HIPCHK(c_, hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, c_->stream));
+        if (out) arrs_.push_back(Arr{p, d, bytes, true});
+        *dev = d;
+        return OHC_OK;
+    }
+    int finish() {
+        if (c_->opts.device_pointers) return OHC_OK;
+        for (auto &a : arrs_)
+            HIPCHK(c_, hipMemcpyAsync(const_cast<void *>(a.host), a.dev, a.bytes,
+                                      hipMemcpyDeviceToHost, c_->stream));
+        HIPCHK(c_, hipStreamSynchronize(c_->stream));
+        return OHC_OK;
+    }
+
+   private:
+    ohc_ctx *c_;
+    size_t slot_ = 0;
+    std::vector<Arr> arrs_;
+};
+
+int check_ctx(ohc_ctx *c);
+int read_stats(ohc_ctx *c, ohc_stats *st);
+// chemistrySource::solve (mode 0) / solveUpdate (mode 1) over n cells; with a
+// device list of cell indices only those nlist cells are integrated.
+int chem_integrate(ohc_ctx *c, int mode, int64_t n, const double *T, double *Tio,
+                   const double *p, const double *rho, const double *Y, double *Yio,
+                   int64_t ldY, const double *dt, double dtFactor, double *subDt, double *R,
+                   int64_t ldR, ohc_stats *stats, const int *list, int64_t nlist);
+
+}  // namespace ohci

@@ -49,6 +49,7 @@ struct LaneParams {
     unsigned long long *counter;  // work-queue head, zeroed before launch
     unsigned long long *stats;    // see ohc_stats: [0..5] counters, [6] dt_min bits, [7] failures
     int exact_dkp;                // 1: exact d(Kp)/dT in the Jacobian
+    int prec;                     // 1: FP32 rates / Gibbs / equilibrium (split precision)
 };
 
 enum : int { PH_NEED = 0, PH_NEWTON = 1, PH_ERR0 = 2, PH_DONE = 3 };
@@ -69,7 +70,7 @@ __device__ __forceinline__ long long wave_max(long long v) {
     return v;
 }
 
-template <class M, int FLOW>
+template <class M, int FLOW, class F = double>
 __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
     constexpr int NS = M::NS, N = NS + 1;
     const int lane = threadIdx.x;
@@ -328,7 +329,7 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
             double yk[N];
 #pragma unroll
             for (int i = 0; i < N; ++i) yk[i] = YK(i);
-            eval_fJ<M, true, FLOW>(yk, pr, f, J, P.exact_dkp != 0);
+            eval_fJ<M, true, FLOW, F>(yk, pr, f, J, P.exact_dkp != 0);
         }
 
         bool have = false;
@@ -453,17 +454,18 @@ struct EvalParams {
     int64_t ldJ;
     int flow;
     int exact;  // 1: exact d(Kp)/dT
+    int prec;   // 1: split precision
 };
 
-template <class M>
+template <class M, class F = double>
 __global__ __launch_bounds__(64) void eval_lane_kernel(const EvalParams P) {
     constexpr int NS = M::NS, N = NS + 1;
     const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= P.n) return;
     double y[N], f[N], J[N][N];
     sfor<0, N>([&](auto i) { y[i] = P.y[i * P.ldy + cell]; });
-    if (P.flow == 0) eval_fJ<M, true, 0>(y, P.pr[cell], f, J, P.exact != 0);
-    else eval_fJ<M, true, 1>(y, P.pr[cell], f, J, P.exact != 0);
+    if (P.flow == 0) eval_fJ<M, true, 0, F>(y, P.pr[cell], f, J, P.exact != 0);
+    else eval_fJ<M, true, 1, F>(y, P.pr[cell], f, J, P.exact != 0);
     sfor<0, N>([&](auto i) { P.f[i * P.ldf + cell] = f[i]; });
     if (P.J)
         sfor<0, N>([&](auto i) {
@@ -479,9 +481,13 @@ struct PropParams {
     double *mu, *kappa, *Dk;
     int64_t ldD;
     double *cp, *ha, *hk;
+    int prec;  // 1: species transport and mixture rules in FP32
 };
 
-template <class M>
+// F = float evaluates the kinetic-theory species properties, Wilke's mixing
+// rule and the binary / mixture-averaged diffusivities in FP32; thermo (cp,
+// h) and the mole fractions stay FP64.
+template <class M, class F = double>
 __global__ __launch_bounds__(256) void props_lane_kernel(const PropParams P) {
     constexpr int NS = M::NS;
     const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -520,54 +526,116 @@ __global__ __launch_bounds__(256) void props_lane_kernel(const PropParams P) {
         }
     }
     // kineticTheory::mu / kappa (kineticTheory.cpp:37-57, kineticTheory.inl:32-34)
+    const F TF = (F)T;
+    F muF[NS], kaF[NS], xF[NS];
     sfor<0, NS>([&](auto k) {
-        const double Ts = T * (1.0 / M::EKB[k]);
-        const double om = 1.147 * exp(-0.145 * log(Ts)) + 1.0 / ((Ts + 0.5) * (Ts + 0.5));
-        mui[k] = 2.6693e-6 * sqrt(M::W[k] * T) / (M::SIGMA[k] * M::SIGMA[k] * om);
-        kap[k] = mui[k] * M::Ri[k] * (cpi[k] / M::Ri[k] + 1.25);
+        xF[k] = (F)xi[k];
+        const F Ts = TF * (F)(1.0 / M::EKB[k]);
+        const F om = (F)1.147 * exp((F)-0.145 * log(Ts)) + (F)1.0 / ((Ts + (F)0.5) * (Ts + (F)0.5));
+        muF[k] = (F)2.6693e-6 * sqrt((F)M::W[k] * TF) / ((F)(M::SIGMA[k] * M::SIGMA[k]) * om);
+        kaF[k] = muF[k] * (F)M::Ri[k] * ((F)cpi[k] / (F)M::Ri[k] + (F)1.25);
     });
     if (P.mu || P.kappa) {
-        double mum = 0.0, km = 0.0;
+        F mum = 0, km = 0;
         if constexpr (NS == 1) {
-            mum = mui[0];
-            km = kap[0];
+            mum = muF[0];
+            km = kaF[0];
         } else {
             // Wilke (transportList.cpp:309-333)
             sfor<0, NS>([&](auto i) {
-                double phi = 0.0;
+                F phi = 0;
                 sfor<0, NS>([&](auto j) {
                     constexpr double Wi = M::W[i], Wj = M::W[j];
-                    const double phiP = (1.0 / sqrt(8.0)) / sqrt(1.0 + Wi / Wj);
-                    const double tt = 1.0 + sqrt(mui[i] / mui[j] * sqrt(Wj / Wi));
-                    phi += phiP * xi[j] * (tt * tt);
+                    constexpr F phiP = (F)(1.0 / ohcd::csqrt(8.0) / ohcd::csqrt(1.0 + Wi / Wj));
+                    constexpr F wr = (F)ohcd::csqrt(Wj / Wi);
+                    const F tt = (F)1.0 + sqrt(muF[i] / muF[j] * wr);
+                    phi += phiP * xF[j] * (tt * tt);
                 });
-                const double rphi = 1.0 / phi;
-                mum += mui[i] * xi[i] * rphi;
-                km += kap[i] * xi[i] * rphi;
+                const F rphi = (F)1.0 / phi;
+                mum += muF[i] * xF[i] * rphi;
+                km += kaF[i] * xF[i] * rphi;
             });
         }
-        if (P.mu) P.mu[cell] = mum;
-        if (P.kappa) P.kappa[cell] = km;
+        if (P.mu) P.mu[cell] = (double)mum;
+        if (P.kappa) P.kappa[cell] = (double)km;
     }
     if (P.Dk) {
         // kineticTheory::Di (kineticTheory.cpp:118-168)
-        const double Tv1p5 = T * sqrt(T), rp = 1.0 / p;
+        const F Tv1p5 = TF * sqrt(TF), rp = (F)(1.0 / p);
         sfor<0, NS>([&](auto i) {
-            double sum = 0.0;
+            F sum = 0;
             sfor<0, NS>([&](auto j) {
                 if constexpr (i != j) {
                     constexpr double Wi = M::W[i], Wj = M::W[j];
-                    const double sij = 0.5 * (M::SIGMA[i] + M::SIGMA[j]);
-                    const double cst = 1.858e-7 * sqrt(1.0 / Wi + 1.0 / Wj) / (sij * sij) * kPatm;
-                    const double Ts = T * (1.0 / sqrt(M::EKB[i] * M::EKB[j]));
-                    const double Od = exp(-0.145 * log(Ts)) + 1.0 / ((Ts + 0.5) * (Ts + 0.5));
-                    const double Dij = Tv1p5 * cst * rp / Od;
-                    sum += xi[j] / Dij;
+                    constexpr double sij = 0.5 * (M::SIGMA[i] + M::SIGMA[j]);
+                    constexpr F cst = (F)(1.858e-7 * ohcd::csqrt(1.0 / Wi + 1.0 / Wj) / (sij * sij) * kPatm);
+                    constexpr F rek = (F)(1.0 / ohcd::csqrt(M::EKB[i] * M::EKB[j]));
+                    const F Ts = TF * rek;
+                    const F Od = exp((F)-0.145 * log(Ts)) + (F)1.0 / ((Ts + (F)0.5) * (Ts + (F)0.5));
+                    const F Dij = Tv1p5 * cst * rp / Od;
+                    sum += xF[j] / Dij;
                 }
             });
-            double Dim = xi[i] < 1.0 - kVeryTiny ? (1.0 - xi[i]) / sum : 0.0;
+            double Dim = xi[i] < 1.0 - kVeryTiny ? (1.0 - xi[i]) / (double)sum : 0.0;
             if (isinf(Dim)) Dim = 0.0;
             P.Dk[i * P.ldD + cell] = Dim;
+        });
+    }
+}
+
+// ---- source terms: calculateSourceTerms / calculateSourceTermsImp ----
+// R_k = W_k omega_k(rho Y / W, T) and, with diag, the diagonal
+// d(omega_k)/d(c_k) plus the temperature coupling at constant internal energy
+// d(omega_k)/dT * W_k (-e_k) / (rho cv) (chemistrySource.cpp:1371-1450,
+// :319-480 with the diagonal of reactionTypes::DqDci, :447-584).  With
+// withoutLast the last species is eliminated through sum Y = 1
+// (omegaCoupled): the diagonal of species i < ns-1 becomes
+// J_ii - J_i,last W_i / W_last + dw_i/dT W_i (e_last - e_i) / (rho cv).
+struct SourceParams {
+    int64_t n;
+    const double *T, *p, *rho, *Y;
+    int64_t ldY;
+    double *R;
+    int64_t ldR;
+    double *D;  // nullable
+    int64_t ldD;
+    int without_last;
+    int exact;
+};
+
+template <class M>
+__global__ __launch_bounds__(64) void source_lane_kernel(const SourceParams P) {
+    constexpr int NS = M::NS, N = NS + 1;
+    const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= P.n) return;
+    const double T = P.T[cell], rho = P.rho[cell];
+    double y[N], f[N], J[N][N], Yk[NS];
+    sfor<0, NS>([&](auto k) {
+        Yk[k] = P.Y[k * P.ldY + cell];
+        y[k] = rho * Yk[k] / M::W[k];
+    });
+    y[NS] = T;
+    eval_fJ<M, true, 0>(y, P.p[cell], f, J, P.exact != 0);
+    sfor<0, NS>([&](auto k) { P.R[k * P.ldR + cell] = f[k] * M::W[k]; });
+    if (!P.D) return;
+    // e_k = ha_k - R_k T per unit mass; cv = sum Y_k (cp_k - R_k)
+    double ea[NS], cv = 0.0;
+    sfor<0, NS>([&](auto k) {
+        double Cp, H, dCp;
+        species_CH<M, k>(T, Cp, H, dCp);
+        ea[k] = H / M::W[k] - M::Ri[k] * T;
+        cv += (Cp / M::W[k] - M::Ri[k]) * Yk[k];
+    });
+    const double b = 1.0 / (rho * cv);
+    if (!P.without_last) {
+        sfor<0, NS>([&](auto i) {
+            P.D[i * P.ldD + cell] = J[i][i] + J[i][NS] * M::W[i] * (-ea[i]) * b;
+        });
+    } else {
+        constexpr int L = NS - 1;
+        sfor<0, L>([&](auto i) {
+            P.D[i * P.ldD + cell] = J[i][i] - J[i][L] * (M::W[i] / M::W[L]) +
+                                    J[i][NS] * M::W[i] * (ea[L] - ea[i]) * b;
         });
     }
 }
@@ -630,23 +698,31 @@ struct LaneEntry {
     hipError_t (*props)(const PropParams &, hipStream_t);
     hipError_t (*tfromh)(const ThParams &, hipStream_t);
     const void *integrate_fn;
+    hipError_t (*source)(const SourceParams &, hipStream_t);
 };
 
 template <class M>
 struct LaneLaunch {
     static hipError_t integrate(const LaneParams &p, int grid, hipStream_t s) {
-        if (p.flow == 0) hipLaunchKernelGGL((ie_lane_kernel<M, 0>), dim3(grid), dim3(64), 0, s, p);
-        else hipLaunchKernelGGL((ie_lane_kernel<M, 1>), dim3(grid), dim3(64), 0, s, p);
+        if (p.prec == 0) {
+            if (p.flow == 0) hipLaunchKernelGGL((ie_lane_kernel<M, 0>), dim3(grid), dim3(64), 0, s, p);
+            else hipLaunchKernelGGL((ie_lane_kernel<M, 1>), dim3(grid), dim3(64), 0, s, p);
+        } else {
+            if (p.flow == 0) hipLaunchKernelGGL((ie_lane_kernel<M, 0, float>), dim3(grid), dim3(64), 0, s, p);
+            else hipLaunchKernelGGL((ie_lane_kernel<M, 1, float>), dim3(grid), dim3(64), 0, s, p);
+        }
         return hipGetLastError();
     }
     static hipError_t eval(const EvalParams &p, hipStream_t s) {
         const int g = (int)((p.n + 63) / 64);
-        hipLaunchKernelGGL(eval_lane_kernel<M>, dim3(g), dim3(64), 0, s, p);
+        if (p.prec == 0) hipLaunchKernelGGL((eval_lane_kernel<M>), dim3(g), dim3(64), 0, s, p);
+        else hipLaunchKernelGGL((eval_lane_kernel<M, float>), dim3(g), dim3(64), 0, s, p);
         return hipGetLastError();
     }
     static hipError_t props(const PropParams &p, hipStream_t s) {
         const int g = (int)((p.n + 255) / 256);
-        hipLaunchKernelGGL(props_lane_kernel<M>, dim3(g), dim3(256), 0, s, p);
+        if (p.prec == 0) hipLaunchKernelGGL((props_lane_kernel<M>), dim3(g), dim3(256), 0, s, p);
+        else hipLaunchKernelGGL((props_lane_kernel<M, float>), dim3(g), dim3(256), 0, s, p);
         return hipGetLastError();
     }
     static hipError_t tfromh(const ThParams &p, hipStream_t s) {
@@ -654,9 +730,15 @@ struct LaneLaunch {
         hipLaunchKernelGGL(tfromh_lane_kernel<M>, dim3(g), dim3(256), 0, s, p);
         return hipGetLastError();
     }
+    static hipError_t source(const SourceParams &p, hipStream_t s) {
+        const int g = (int)((p.n + 63) / 64);
+        hipLaunchKernelGGL(source_lane_kernel<M>, dim3(g), dim3(64), 0, s, p);
+        return hipGetLastError();
+    }
     static LaneEntry entry(const char *name) {
-        return LaneEntry{name,      M::HASH,      M::THASH, M::NS, &integrate, &eval,
-                         &props,    &tfromh,      (const void *)&ie_lane_kernel<M, 0>};
+        return LaneEntry{name,    M::HASH, M::THASH, M::NS,
+                         &integrate, &eval,  &props,   &tfromh,
+                         (const void *)&ie_lane_kernel<M, 0>, &source};
     }
 };
 

@@ -23,35 +23,9 @@
 #include "wave_dispatch.hpp"
 #include "wave_kernels.hpp"
 
-struct ohc_mech {
-    ohc::Mechanism m;
-};
+#include "ctx_internal.hpp"
 
-struct ohc_ctx {
-    ohc::Mechanism mech;
-    ohc_opts opts;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    const ohcd::LaneEntry *lane = nullptr;  // specialised kernels for this mechanism
-    bool lane_transport = false;
-    bool wave = false;          // generic wavefront-per-cell kernels (run-time tables)
-    ohcd::WMech wm{};           // device tables of the generic kernels
-    void *d_wm = nullptr;       // one allocation holding them
-    int grid = 0;  // persistent integrator grid (single-wavefront blocks)
-    unsigned long long *d_ctl = nullptr;  // [0] queue head, [1..8] stats
-    std::vector<void *> stage;            // host-pointer staging buffers
-    std::vector<size_t> stage_sz;
-    // longest-first schedule: per-cell cost of the last call and its order
-    int64_t sched_n = 0;
-    bool sched_ready = false;
-    int *d_cost = nullptr, *d_order = nullptr, *d_iota = nullptr, *d_keys = nullptr;
-    void *d_sort_tmp = nullptr;
-    size_t sort_tmp_bytes = 0;
-    std::string err, kname;
-};
-
-namespace {
+namespace ohci {
 
 thread_local std::string g_err;
 
@@ -60,14 +34,6 @@ int fail(ohc_ctx *c, int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
-
-#define HIPCHK(ctx, expr)                                                                   \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail(ctx, OHC_ERR_DEVICE,                                                \
-                        std::string(#expr) + ": " + hipGetErrorString(e_));                 \
-    } while (0)
 
 __global__ void reset_ctl(unsigned long long *ctl) {
     const int i = threadIdx.x;
@@ -82,70 +48,6 @@ bool read_file(const char *path, std::string &out) {
     out = s.str();
     return true;
 }
-
-// Device staging for host pointers: slot i holds at least bytes.
-int stage_buf(ohc_ctx *c, size_t slot, size_t bytes, void **out) {
-    if (c->stage.size() <= slot) {
-        c->stage.resize(slot + 1, nullptr);
-        c->stage_sz.resize(slot + 1, 0);
-    }
-    if (c->stage_sz[slot] < bytes) {
-        if (c->stage[slot]) {
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            HIPCHK(c, hipFree(c->stage[slot]));
-        }
-        c->stage[slot] = nullptr;
-        c->stage_sz[slot] = 0;
-        HIPCHK(c, hipMalloc(&c->stage[slot], bytes));
-        c->stage_sz[slot] = bytes;
-    }
-    *out = c->stage[slot];
-    return OHC_OK;
-}
-
-// Bring one array to the device (host mode) or pass it through (device mode).
-struct Arr {
-    const void *host = nullptr;
-    void *dev = nullptr;
-    size_t bytes = 0;
-    bool out = false;
-};
-
-class Staging {
-   public:
-    explicit Staging(ohc_ctx *c) : c_(c) {}
-    // in: copy host->device; out: copy device->host at finish()
-    int map(const void *p, size_t bytes, bool in, bool out, void **dev) {
-        if (!p) {
-            *dev = nullptr;
-            return OHC_OK;
-        }
-        if (c_->opts.device_pointers) {
-            *dev = const_cast<void *>(p);
-            return OHC_OK;
-        }
-        void *d = nullptr;
-        int rc = stage_buf(c_, slot_++, bytes, &d);
-        if (rc) return rc;
-        if (in) HIPCHK(c_, hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, c_->stream));
-        if (out) arrs_.push_back(Arr{p, d, bytes, true});
-        *dev = d;
-        return OHC_OK;
-    }
-    int finish() {
-        if (c_->opts.device_pointers) return OHC_OK;
-        for (auto &a : arrs_)
-            HIPCHK(c_, hipMemcpyAsync(const_cast<void *>(a.host), a.dev, a.bytes,
-                                      hipMemcpyDeviceToHost, c_->stream));
-        HIPCHK(c_, hipStreamSynchronize(c_->stream));
-        return OHC_OK;
-    }
-
-   private:
-    ohc_ctx *c_;
-    size_t slot_ = 0;
-    std::vector<Arr> arrs_;
-};
 
 int check_ctx(ohc_ctx *c) {
     if (!c) return fail(nullptr, OHC_ERR_ARG, "null context");
@@ -317,7 +219,7 @@ int read_stats(ohc_ctx *c, ohc_stats *st) {
 int chem_integrate(ohc_ctx *c, int mode, int64_t n, const double *T, double *Tio,
                    const double *p, const double *rho, const double *Y, double *Yio,
                    int64_t ldY, const double *dt, double dtFactor, double *subDt, double *R,
-                   int64_t ldR, ohc_stats *stats) {
+                   int64_t ldR, ohc_stats *stats, const int *list, int64_t nlist) {
     int rc = check_ctx(c);
     if (rc) return rc;
     const int ns = c->mech.ns;
@@ -383,7 +285,11 @@ int chem_integrate(ohc_ctx *c, int mode, int64_t n, const double *T, double *Tio
     P.Tbeta = o.T_beta;
     P.dt_max = o.dt_max;
     P.max_cell_iters = 50000000;
-    if (o.schedule) {
+    if (list) {
+        // integrate only the listed cells (device array of cell indices)
+        P.n = nlist;
+        P.order = list;
+    } else if (o.schedule) {
         if ((rc = ensure_sched(c, n))) return rc;
         P.order = c->sched_ready ? c->d_order : nullptr;
         P.cost = c->d_cost;
@@ -391,20 +297,23 @@ int chem_integrate(ohc_ctx *c, int mode, int64_t n, const double *T, double *Tio
     P.counter = c->d_ctl;
     P.stats = c->d_ctl + 1;
     P.exact_dkp = o.jacobian;
+    P.prec = o.precision;
     hipLaunchKernelGGL(reset_ctl, dim3(1), dim3(64), 0, c->stream, c->d_ctl);
     HIPCHK(c, hipGetLastError());
-    if (c->lane) {
-        const int grid = (int)std::min<int64_t>(c->grid, (n + 63) / 64);
-        HIPCHK(c, c->lane->integrate(P, grid, c->stream));
-    } else {
-        ohcd::WaveParams W;
-        W.L = P;
-        W.M = c->wm;
-        W.exact_dkp = o.jacobian;
-        const int grid = (int)std::min<int64_t>(c->grid, n);
-        HIPCHK(c, ohcd::wave_integrate(W, grid, c->stream));
+    if (P.n > 0) {
+        if (c->lane) {
+            const int grid = (int)std::min<int64_t>(c->grid, (P.n + 63) / 64);
+            HIPCHK(c, c->lane->integrate(P, grid, c->stream));
+        } else {
+            ohcd::WaveParams W;
+            W.L = P;
+            W.M = c->wm;
+            W.exact_dkp = o.jacobian;
+            const int grid = (int)std::min<int64_t>(c->grid, P.n);
+            HIPCHK(c, ohcd::wave_integrate(W, grid, c->stream));
+        }
     }
-    if (o.schedule) {
+    if (o.schedule && !list) {
         // next call on these cells: most expensive first
         size_t tmp = c->sort_tmp_bytes;
         HIPCHK(c, hipcub::DeviceRadixSort::SortPairsDescending(c->d_sort_tmp, tmp, c->d_cost,
@@ -417,7 +326,9 @@ int chem_integrate(ohc_ctx *c, int mode, int64_t n, const double *T, double *Tio
     return OHC_OK;
 }
 
-}  // namespace
+}  // namespace ohci
+
+using namespace ohci;
 
 extern "C" {
 
@@ -531,7 +442,7 @@ int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_
     if (!(o.safe_scale > 0) || !(o.min_scale > 0) || !(o.max_scale > 0) || !(o.alpha_inc > 0) ||
         !(o.T_beta > 0) || !(o.T_beta < 1))
         return fail(nullptr, OHC_ERR_ARG, "step-control factors out of range");
-    if (o.precision != 0) return fail(nullptr, OHC_ERR_UNSUPPORTED, "precision mode not available");
+    if (o.precision != 0 && o.precision != 1) return fail(nullptr, OHC_ERR_ARG, "precision must be 0 or 1");
     if (o.jacobian != 0 && o.jacobian != 1) return fail(nullptr, OHC_ERR_ARG, "jacobian must be 0 or 1");
     if (o.kernel < 0 || o.kernel > 2) return fail(nullptr, OHC_ERR_ARG, "kernel must be 0, 1 or 2");
     int ndev = 0;
@@ -557,6 +468,9 @@ int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_
             c->lane_transport = th != 0 && ent[i].thash == th;
         }
     }
+    if (!c->lane && o.precision == 1)
+        return fail(nullptr, OHC_ERR_UNSUPPORTED,
+                    "the FP32 split precision mode needs a specialised kernel for this mechanism");
     if (!c->lane && o.kernel == 1)
         return fail(nullptr, OHC_ERR_UNSUPPORTED,
                     "no specialised kernel compiled for this mechanism; compiled: " +
@@ -621,24 +535,45 @@ int ohc_chem_solve(ohc_ctx *c, int64_t n, const double *T, const double *p, cons
                    const double *Y, int64_t ldY, const double *dt, double dtFactor,
                    double *subDt, double *R, int64_t ldR, ohc_stats *stats) {
     return chem_integrate(c, 0, n, T, nullptr, p, rho, Y, nullptr, ldY, dt, dtFactor, subDt, R,
-                          ldR, stats);
+                          ldR, stats, nullptr, 0);
 }
 
 int ohc_chem_solve_update(ohc_ctx *c, int64_t n, double *T, const double *p, const double *rho,
                           double *Y, int64_t ldY, const double *dt, double dtFactor,
                           double *subDt, ohc_stats *stats) {
     return chem_integrate(c, 1, n, nullptr, T, p, rho, nullptr, Y, ldY, dt, dtFactor, subDt,
-                          nullptr, n, stats);
+                          nullptr, n, stats, nullptr, 0);
 }
 
 int ohc_chem_source(ohc_ctx *c, int64_t n, const double *T, const double *p, const double *rho,
                     const double *Y, int64_t ldY, double *R, int64_t ldR, double *diagJ,
                     int64_t ldD, int32_t withoutLastSpec) {
-    (void)n, (void)T, (void)p, (void)rho, (void)Y, (void)ldY, (void)R, (void)ldR, (void)diagJ,
-        (void)ldD, (void)withoutLastSpec;
     int rc = check_ctx(c);
     if (rc) return rc;
-    return fail(c, OHC_ERR_UNSUPPORTED, "ohc_chem_source is not available in this build");
+    const int ns = c->mech.ns;
+    if (n < 0 || ldY < n || ldR < n || (diagJ && ldD < n)) return fail(c, OHC_ERR_ARG, "bad sizes");
+    if (!T || !p || !rho || !Y || !R) return fail(c, OHC_ERR_ARG, "null array argument");
+    if (withoutLastSpec != 0 && withoutLastSpec != 1)
+        return fail(c, OHC_ERR_ARG, "withoutLastSpec must be 0 or 1");
+    if (n == 0) return OHC_OK;
+    if (!c->lane && !c->wave) return fail(c, OHC_ERR_UNSUPPORTED, "no device kernels for this mechanism");
+    if (!c->lane && n > 0x7fffffffll) return fail(c, OHC_ERR_ARG, "n too large");
+    Staging st(c);
+    const size_t sv = sizeof(double) * (size_t)n;
+    void *dT, *dp, *drho, *dY, *dR, *dD = nullptr;
+    if ((rc = st.map(T, sv, true, false, &dT))) return rc;
+    if ((rc = st.map(p, sv, true, false, &dp))) return rc;
+    if ((rc = st.map(rho, sv, true, false, &drho))) return rc;
+    if ((rc = st.map(Y, sizeof(double) * (size_t)(ldY * (ns - 1) + n), true, false, &dY))) return rc;
+    if ((rc = st.map(R, sizeof(double) * (size_t)(ldR * (ns - 1) + n), false, true, &dR))) return rc;
+    if (diagJ && (rc = st.map(diagJ, sizeof(double) * (size_t)(ldD * (ns - 1) + n), false, true, &dD)))
+        return rc;
+    ohcd::SourceParams P{n,   (const double *)dT, (const double *)dp, (const double *)drho,
+                         (const double *)dY, ldY, (double *)dR, ldR, (double *)dD, ldD,
+                         withoutLastSpec, c->opts.jacobian};
+    if (c->lane) HIPCHK(c, c->lane->source(P, c->stream));
+    else HIPCHK(c, ohcd::wave_source(P, c->wm, c->stream));
+    return st.finish();
 }
 
 static int eval_common(ohc_ctx *c, int64_t n, const double *y, int64_t ldy, const double *p,
@@ -659,7 +594,8 @@ static int eval_common(ohc_ctx *c, int64_t n, const double *y, int64_t ldy, cons
     if (J && (rc = st.map(J, sizeof(double) * (size_t)(ldJ * (N * N - 1) + n), false, true, &dJ)))
         return rc;
     ohcd::EvalParams P{n,     (const double *)dy, ldy, (const double *)dp, (double *)df,
-                       ldf,   (double *)dJ,       ldJ, c->opts.flow_type,  c->opts.jacobian};
+                       ldf,   (double *)dJ,       ldJ, c->opts.flow_type,  c->opts.jacobian,
+                       c->opts.precision};
     if (c->lane) HIPCHK(c, c->lane->eval(P, c->stream));
     else HIPCHK(c, ohcd::wave_eval(P, c->wm, c->opts.jacobian, c->stream));
     return st.finish();
@@ -726,7 +662,8 @@ int ohc_mix_props(ohc_ctx *c, int64_t n, const double *T, const double *p, const
     if ((rc = st.map(hk, sD, false, true, &dhk))) return rc;
     ohcd::PropParams P{n,          (const double *)dT, (const double *)dp, (const double *)dY,
                        ldY,        (double *)dmu,      (double *)dk,       (double *)dD,
-                       ldD,        (double *)dcp,      (double *)dha,      (double *)dhk};
+                       ldD,        (double *)dcp,      (double *)dha,      (double *)dhk,
+                       c->opts.precision};
     HIPCHK(c, c->lane->props(P, c->stream));
     return st.finish();
 }

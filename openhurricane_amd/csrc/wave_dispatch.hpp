@@ -14,6 +14,7 @@ struct WaveParams;
 int wave_size(int n);
 hipError_t wave_integrate(const WaveParams &p, int grid, hipStream_t s);
 hipError_t wave_eval(const EvalParams &p, const WMech &m, int exact, hipStream_t s);
+hipError_t wave_source(const SourceParams &p, const WMech &m, hipStream_t s);
 const void *wave_integrate_fn(int n);
 
 }  // namespace ohcd

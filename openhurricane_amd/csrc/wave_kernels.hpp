@@ -823,6 +823,38 @@ __global__ __launch_bounds__(64) void ie_wave_kernel(const WaveParams WP) {
     }
 }
 
+// Source terms and their (coupled) diagonal, one wavefront per cell; same
+// definitions as source_lane_kernel.
+template <int N>
+__global__ __launch_bounds__(64) void source_wave_kernel(const SourceParams P, const WMech M) {
+    __shared__ WaveLDS<N> L;
+    const int lane = threadIdx.x, ns = M.ns;
+    const int64_t cell = blockIdx.x;
+    if (cell >= P.n) return;
+    const bool sp = lane < ns;
+    const double T = P.T[cell], rho = P.rho[cell];
+    const double W = sp ? M.sp[SP_W * 64 + lane] : 1.0;
+    const double Yk = sp ? P.Y[lane * P.ldY + cell] : 0.0;
+    const double y = sp ? rho * Yk / W : (lane == ns ? T : 0.0);
+    const double w = wave_eval<N, 0, true>(M, L, y, P.exact);
+    if (sp) P.R[lane * P.ldR + cell] = w * W;
+    if (!P.D) return;
+    const double Ri = kRu / W;
+    const double ea = sp ? L.H[lane] / W - Ri * T : 0.0;
+    const double cv = wsum(sp ? (L.Cp[lane] / W - Ri) * Yk : 0.0);
+    const double b = 1.0 / (rho * cv);
+    const double *row = L.J + lane * (N + 1);
+    if (!P.without_last) {
+        if (sp) P.D[lane * P.ldD + cell] = row[lane] + row[ns] * W * (-ea) * b;
+    } else {
+        const int l = ns - 1;
+        const double Wl = M.sp[SP_W * 64 + l];
+        const double el = rdlane(ea, l);
+        if (lane < l)
+            P.D[lane * P.ldD + cell] = row[lane] - row[l] * (W / Wl) + row[ns] * W * (el - ea) * b;
+    }
+}
+
 // dy/dt and J at given states, one wavefront per cell.
 template <int N, int FLOW>
 __global__ __launch_bounds__(64) void eval_wave_kernel(const EvalParams P, const WMech M, int exact) {

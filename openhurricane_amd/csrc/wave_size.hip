@@ -27,6 +27,12 @@ hipError_t OHC_CAT(wave_eval_, OHC_WAVE_N)(const EvalParams &p, const WMech &m, 
     return hipGetLastError();
 }
 
+hipError_t OHC_CAT(wave_source_, OHC_WAVE_N)(const SourceParams &p, const WMech &m,
+                                              hipStream_t s) {
+    hipLaunchKernelGGL((source_wave_kernel<OHC_WAVE_N>), dim3((unsigned)p.n), dim3(64), 0, s, p, m);
+    return hipGetLastError();
+}
+
 const void *OHC_CAT(wave_fn_, OHC_WAVE_N)() {
     return (const void *)&ie_wave_kernel<OHC_WAVE_N, 0>;
 }

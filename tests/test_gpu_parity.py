@@ -167,3 +167,58 @@ def test_longest_first_schedule_is_bitwise_neutral(h2mech, c1opts, states):
         np.testing.assert_array_equal(Y, outs[0][1])
         np.testing.assert_array_equal(sub, outs[0][2])
         assert s["newton_iters"] == outs[0][3]["newton_iters"]
+
+
+@pytest.mark.parametrize("without_last", [0, 1])
+def test_source_terms_match_oracle(cs, h2oracle, states, c1opts, without_last):
+    """calculateSourceTerms / calculateSourceTermsImp: R_k and the (coupled)
+    diagonal against the oracle; R to 1e-9 of the cell's largest |R|, the
+    diagonal to 1e-9 of its largest entry."""
+    st, t = states
+    n = 1024
+    T, p, rho = (np.ascontiguousarray(st[k][:n]) for k in ("T", "p", "rho"))
+    Y = np.ascontiguousarray(st["Y"][:, :n])
+    R, D = cs.source_terms(T, p, rho, Y, diag=True, without_last=bool(without_last))
+    Ro, Do = h2oracle.chem_source(c1opts, T, p, rho, Y, diag=True, without_last=bool(without_last))
+    assert np.all(np.abs(R - Ro) <= 1e-9 * np.abs(Ro).max(axis=0) + 1e-300)
+    nd = 9 - without_last
+    sc = np.abs(Do[:nd]).max(axis=0) + 1e-300
+    assert np.all(np.abs(D[:nd] - Do[:nd]) <= 1e-9 * sc)
+    R2 = cs.source_terms(T, p, rho, Y)
+    np.testing.assert_array_equal(R, R2)
+
+
+def test_split_precision_errors(h2mech, states, c1opts):
+    """FP32 Gibbs / equilibrium / rate constants and FP32 transport against the
+    FP64 path on the same cells: per-output max relative error (printed) within
+    the stated bounds -- integrated T 1e-6, species with Y >= 1e-3 1e-4,
+    RHS 1e-4 of the cell's largest rate, viscosity / conductivity / D_k 1e-5."""
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource
+
+    st, t = states
+    n = 2048
+    kw = dict(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6)
+    c64 = ChemistrySource(h2mech, opts=_lib.default_opts(**kw))
+    c32 = ChemistrySource(h2mech, opts=_lib.default_opts(precision=1, **kw))
+    dt = np.full(n, 1e-6)
+    out = {}
+    for name, c in (("f64", c64), ("f32", c32)):
+        T, Y = st["T"].copy(), st["Y"].copy()
+        c.solve_update(T, st["p"], st["rho"], Y, dt, 1.0)
+        y = np.ascontiguousarray(np.vstack([st["rho"] * st["Y"] / t["W"][:, None], st["T"][None, :]]))
+        f = c.eval_rhs(y, st["p"])
+        props = c.mix_props(T, st["p"], Y)
+        out[name] = (T, Y, f, props)
+    T64, Y64, f64, p64 = out["f64"]
+    T32, Y32, f32, p32 = out["f32"]
+    eT = np.max(np.abs(T32 - T64) / T64)
+    major = Y64 >= 1e-3
+    eY = np.max(np.abs(Y32 - Y64)[major] / Y64[major])
+    ef = np.max(np.abs(f32[:-1] - f64[:-1]) / (np.abs(f64[:-1]).max(axis=0) + 1e-300))
+    ep = {k: np.max(np.abs(p32[k] - p64[k]) / np.abs(p64[k])) for k in ("mu", "kappa")}
+    eD = np.max(np.abs(p32["Dk"] - p64["Dk"]) / np.abs(p64["Dk"]).max(axis=0))
+    print(f"split precision max rel err: T {eT:.2e}  Y(major) {eY:.2e}  rhs {ef:.2e}  "
+          f"mu {ep['mu']:.2e}  kappa {ep['kappa']:.2e}  Dk {eD:.2e}")
+    assert eT <= 1e-6 and eY <= 1e-4 and ef <= 1e-4
+    assert ep["mu"] <= 1e-5 and ep["kappa"] <= 1e-5 and eD <= 1e-5

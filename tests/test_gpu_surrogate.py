@@ -1,0 +1,151 @@
+"""GPU tests of the chemistry surrogate (BF16 MFMA training and inference
+kernels, gate and fallback) through the C-ABI.
+
+  * the device loss gradient equals the CPU reference's (oracle/mlp_ref.py,
+    same BF16 roundings, FP64 sums) -- per layer within 1 % of the gradient
+    norm, loss within 1e-3;
+  * device inference equals the CPU reference within 1 % of each output's
+    spread;
+  * training on pairs produced by the integrator lowers the loss;
+  * cells the gate rejects get exactly the plain integrator's result;
+  * parameters survive save / load.
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a HIP GPU")]
+
+WIDTH, DEPTH = 256, 3
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource, Mechanism
+    from openhurricane_amd.sampler import synthetic_workload
+
+    mech = Mechanism.synthetic(32, 150, seed=1)
+    t = mech.tables()
+    opts = _lib.default_opts(atol=1e-10, rtol=1e-6, atol_inc=1e-7, rtol_inc=1e-3, jacobian=1,
+                             device_pointers=1, kernel=2)
+    cs = ChemistrySource(mech, opts=opts)
+    dev = torch.device("cuda:0")
+    n = 8192
+    st = synthetic_workload(mech.species, t["W"], n, seed=21)
+    g = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in st.items() if k != "kind"}
+    T1, Y1 = g["T"].clone(), g["Y"].clone()
+    dt = torch.full((n,), 1e-6, dtype=torch.float64, device=dev)
+    cs.solve_update(T1, g["p"], g["rho"], Y1, dt, 1.0)
+    return dict(mech=mech, cs=cs, g=g, T1=T1, Y1=Y1, dt=dt, n=n, torch=torch)
+
+
+def _surrogate(env, seed=3):
+    from openhurricane_amd.surrogate import Surrogate
+
+    s = Surrogate(env["cs"], width=WIDTH, depth=DEPTH, seed=seed)
+    g = env["g"]
+    X, Yt = s.features(g["T"], g["p"], g["Y"], env["T1"], env["Y1"])
+    s.fit_norm(X, Yt, 1e-6)
+    return s, X, Yt
+
+
+def test_gradient_matches_cpu_reference(env):
+    from mlp_ref import forward_backward
+
+    s, X, Yt = _surrogate(env)
+    m = 256
+    xb, yb = X[:m].contiguous(), Yt[:m].contiguous()
+    g_dev, loss_dev = s.gradient(xb, yb)
+    in_mean, in_std, _, _, out_mean, out_std, w = s.norm
+    norm = (in_mean.astype(np.float64), in_std.astype(np.float64), out_mean.astype(np.float64),
+            out_std.astype(np.float64), np.ones(s.nout) if w is None else w.astype(np.float64))
+    loss_ref, g_ref, _ = forward_backward(s.get_params().astype(np.float64),
+                                          xb.cpu().numpy().astype(np.float64),
+                                          yb.cpu().numpy().astype(np.float64), norm, s.nin,
+                                          s.nout, WIDTH, DEPTH, bf=True)
+    assert loss_dev == pytest.approx(loss_ref, rel=1e-3)
+    off = 0
+    for l in range(DEPTH + 1):
+        i = s.nin if l == 0 else WIDTH
+        o = WIDTH if l < DEPTH else s.nout
+        for sz in (o * i, o):
+            a, b = g_dev[off:off + sz].astype(np.float64), g_ref[off:off + sz]
+            err = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+            print(f"layer {l} block {sz}: rel err {err:.2e}")
+            assert err < 1e-2, (l, sz, err)
+            off += sz
+    assert off == s.num_params
+
+
+def test_inference_matches_cpu_reference(env):
+    from mlp_ref import predict
+
+    s, X, Yt = _surrogate(env, seed=5)
+    g = env["g"]
+    m = 512
+    out = s.predict(g["T"][:m].contiguous(), g["p"][:m].contiguous(), g["Y"][:, :m].contiguous())
+    in_mean, in_std, _, _, out_mean, out_std, w = s.norm
+    norm = (in_mean.astype(np.float64), in_std.astype(np.float64), out_mean.astype(np.float64),
+            out_std.astype(np.float64), np.ones(s.nout))
+    ref = predict(s.get_params().astype(np.float64), X[:m].cpu().numpy().astype(np.float64), norm,
+                  s.nin, s.nout, WIDTH, DEPTH, bf=True)
+    err = np.abs(out.cpu().numpy() - ref) / out_std[None, :]
+    print("max normalised error", err.max())
+    assert err.max() < 1e-2
+
+
+def test_training_lowers_loss(env):
+    s, X, Yt = _surrogate(env, seed=7)
+    torch = env["torch"]
+    l0 = s.train_step(X[:4096].contiguous(), Yt[:4096].contiguous(), 0.0, want_loss=True)
+    hist = s.fit(X, Yt, epochs=40, batch=1024, lr=2e-3, lr_end=1e-4)
+    print("loss", l0, "->", hist[-1])
+    assert hist[-1] < 0.2 * l0
+    # the learned increments beat "no change" on the training pairs
+    pred = s.predict(env["g"]["T"], env["g"]["p"], env["g"]["Y"])
+    mse_net = ((pred - Yt) ** 2).mean(0)
+    mse_zero = (Yt ** 2).mean(0)
+    assert float((mse_net < mse_zero).float().mean()) > 0.9
+
+
+def test_gate_rejected_cells_match_plain_integrator(env):
+    s, X, Yt = _surrogate(env, seed=9)
+    torch, g, cs = env["torch"], env["g"], env["cs"]
+    n = env["n"]
+    T, Y = g["T"].clone(), g["Y"].clone()
+    sub = torch.zeros(n, dtype=torch.float64, device=T.device)
+    # a time step the network was not trained for: every cell falls back
+    dt2 = torch.full((n,), 2e-6, dtype=torch.float64, device=T.device)
+    nf, st = s.solve_update(T, g["p"], g["rho"], Y, dt2, 1.0, subDt=sub, gate_tol=1.0)
+    assert nf == n and st["cells"] == n
+    T2, Y2 = g["T"].clone(), g["Y"].clone()
+    cs.solve_update(T2, g["p"], g["rho"], Y2, dt2, 1.0)
+    assert torch.equal(T, T2) and torch.equal(Y, Y2)
+    # a zero element tolerance rejects every cell whose prediction moves an element
+    T, Y = g["T"].clone(), g["Y"].clone()
+    nf, st = s.solve_update(T, g["p"], g["rho"], Y, env["dt"], 1.0, gate_tol=0.0)
+    assert nf > 0.9 * n
+    # an open gate takes the network everywhere except outside the feature box
+    T, Y = g["T"].clone(), g["Y"].clone()
+    nf, st = s.solve_update(T, g["p"], g["rho"], Y, env["dt"], 1.0, gate_tol=1e30)
+    assert nf < 0.05 * n
+    np.testing.assert_allclose(Y.sum(0).cpu().numpy(), 1.0, rtol=1e-12)
+
+
+def test_save_load_round_trip(env, tmp_path):
+    s, X, Yt = _surrogate(env, seed=11)
+    s.fit(X[:4096], Yt[:4096], epochs=1, batch=1024)
+    f = tmp_path / "net.bin"
+    s.save(f)
+    s2, _, _ = _surrogate(env, seed=12)
+    s2.load(f)
+    np.testing.assert_array_equal(s.get_params(), s2.get_params())
+    g = env["g"]
+    a = s.predict(g["T"], g["p"], g["Y"])
+    b = s2.predict(g["T"], g["p"], g["Y"])
+    assert env["torch"].equal(a, b)
