@@ -159,6 +159,11 @@ typedef struct ohc_opts {
                                states within the Newton tolerance, far fewer
                                Newton iterations when heat release couples
                                strongly to the rates                              */
+    int32_t integrator;     /* 0: ImplicitEuler (chemistrySource, the CPU path);
+                               1: the reference GPU path -- explicit Euler with a
+                               step-doubling error estimate (EulerCUDA), every
+                               cell integrated, subDt the starting step, Y from
+                               c without clipping; compiled mechanisms only       */
 } ohc_opts;
 
 /* Per-call counters (summed over cells) for roofline accounting. */

@@ -46,7 +46,7 @@ class Opts(C.Structure):
         ("T_beta", C.c_double), ("dt_max", C.c_double), ("T_low", C.c_double),
         ("T_high", C.c_double), ("precision", C.c_int32), ("device_pointers", C.c_int32),
         ("kernel", C.c_int32), ("waves_per_cu", C.c_int32), ("schedule", C.c_int32),
-        ("jacobian", C.c_int32),
+        ("jacobian", C.c_int32), ("integrator", C.c_int32),
     ]
 
 

@@ -50,6 +50,7 @@ struct LaneParams {
     unsigned long long *stats;    // see ohc_stats: [0..5] counters, [6] dt_min bits, [7] failures
     int exact_dkp;                // 1: exact d(Kp)/dT in the Jacobian
     int prec;                     // 1: FP32 rates / Gibbs / equilibrium (split precision)
+    int integrator;               // 0: ImplicitEuler; 1: explicit Euler (reference GPU path)
 };
 
 enum : int { PH_NEED = 0, PH_NEWTON = 1, PH_ERR0 = 2, PH_DONE = 3 };
@@ -442,6 +443,186 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
     }
 }
 
+// ---- the reference GPU integrator (comparison mode) ----
+// Explicit Euler with a step-doubling error estimate, per cell as
+// solveChemODEs (CS/src/CUDAChemistrySourceODEs.cu:588-640) drives
+// EulerCUDA::solve / adaptiveSolve (CS/src/EulerCUDA.cu:34-62, :130-224) with
+// ODEsSolverCUDA::maxError (CS/src/ODEsSolverCUDA.cu:34-48): an attempt of
+// size h from y0 takes two half steps (y_h) and one full step (y_f) with
+// f0 = f(y0); error = max |2 (y_f - y_h)| / (atol + rtol max(|y_f|, |y_h|)).
+// Every cell is integrated (no temperature threshold) starting from subDt.
+// Lane-per-cell state machine: each loop trip evaluates f once per lane, at
+// y0 (start of a step) or at the half-step point of the current attempt.
+constexpr int kEEMaxTinyRejects = 1000;  // rejected attempts at h = 1e-15 before a cell fails
+
+template <class M, int FLOW>
+__global__ __launch_bounds__(64) void ee_lane_kernel(const LaneParams P) {
+    constexpr int NS = M::NS, N = NS + 1;
+    enum { E_NEED, E_F0, E_MID, E_DONE };
+    const int lane = threadIdx.x;
+    __shared__ double s_y0[N][64], s_f0[N][64];
+    int phase = E_NEED, last = 0, nstep = 0, rejects = 0;
+    long long cell = 0, csteps = 0;
+    double pr = 0.0, t = 0.0, dt = 0.0, dtl = 0.0, dtTry = 0.0, tn = 0.0, left = 0.0, sub = 0.0,
+           dtc = 0.0;
+    long long c_cells = 0, c_steps = 0, c_att = 0, c_rhs = 0, c_maxs = 0, c_fail = 0;
+    double dtmin = __builtin_huge_val();
+    const double errg = pow(10.0 / 0.8, -1.0 / 0.25);
+
+    auto finish_cell = [&](bool failed) {
+        P.subDt[cell] = sub;
+        if (P.mode == 0) {
+            const double rho = P.rho[cell], rdt = 1.0 / dtc;
+            sfor<0, NS>([&](auto k) {
+                const double c0 = rho * P.Y[k * P.ldY + cell] / M::W[k];
+                P.R[k * P.ldR + cell] = (s_y0[k][lane] - c0) * rdt * M::W[k];
+            });
+        } else {
+            // device ciToYi (CUDAChemistrySourceODEs.cu:545-586): Y = c W / sum c W
+            double r = 0.0;
+            sfor<0, NS>([&](auto k) { r += s_y0[k][lane] * M::W[k]; });
+            const double rr = 1.0 / r;
+            sfor<0, NS>([&](auto k) { P.Yout[k * P.ldY + cell] = s_y0[k][lane] * M::W[k] * rr; });
+            P.Tout[cell] = s_y0[NS][lane];
+        }
+        if (P.cost) P.cost[cell] = (int)(csteps < 0x7fffffffll ? csteps : 0x7fffffffll);
+        c_cells += 1;
+        c_fail += failed ? 1 : 0;
+        c_maxs = csteps > c_maxs ? csteps : c_maxs;
+        dtmin = fmin(dtmin, sub);
+        phase = E_NEED;
+    };
+    // EulerCUDA::solve loop head; false when the step budget is exhausted
+    auto start_step = [&]() -> bool {
+        if (nstep >= P.max_steps) {
+            tn = t;
+            return false;
+        }
+        dtTry = dt;
+        if ((t + dt - tn) * (t + dt) > 0) {
+            dt = tn - t;
+            last = 1;
+        } else if (t + dt == tn) {
+            last = 1;
+        }
+        dtl = dt;
+        rejects = 0;
+        phase = E_F0;
+        return true;
+    };
+    auto ode_end = [&]() {
+        left -= tn;
+        if (left > 0.0) {
+            tn = left;
+            t = 0.0;
+            dt = sub;
+            last = 0;
+            nstep = 0;
+            if (!start_step()) {
+                // no step possible: tn = t = 0 would not advance; give up on the cell
+                finish_cell(true);
+            }
+        } else {
+            finish_cell(false);
+        }
+    };
+
+    for (;;) {
+        for (;;) {
+            const unsigned long long need = __ballot(phase == E_NEED);
+            if (need == 0ull) break;
+            const int leader = __ffsll((long long)need) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(P.counter, (unsigned long long)__popcll(need));
+            base = __shfl(base, leader);
+            if (phase == E_NEED) {
+                const unsigned long long below = lane ? (need & ((1ull << lane) - 1ull)) : 0ull;
+                const unsigned long long mine = base + (unsigned long long)__popcll(below);
+                if (mine >= (unsigned long long)P.n) {
+                    phase = E_DONE;
+                } else {
+                    cell = P.order ? (long long)P.order[mine] : (long long)mine;
+                    const double rho = P.rho[cell];
+                    pr = FLOW == 0 ? P.p[cell] : rho;
+                    dtc = P.dt[cell] * P.dtFactor;
+                    const double s0 = P.subDt[cell];
+                    sub = s0 > 0.0 ? s0 : dtc;
+                    sfor<0, NS>([&](auto k) { s_y0[k][lane] = rho * P.Y[k * P.ldY + cell] / M::W[k]; });
+                    s_y0[NS][lane] = P.T[cell];
+                    csteps = 0;
+                    left = dtc;
+                    tn = left;
+                    t = 0.0;
+                    dt = sub;
+                    last = 0;
+                    nstep = 0;
+                    if (!start_step()) finish_cell(true);
+                }
+            }
+        }
+        if (!__any(phase == E_F0 || phase == E_MID)) break;
+        double y[N], f[N], Jd[N][N];
+        sfor<0, N>([&](auto i) {
+            const double y0 = s_y0[i][lane];
+            y[i] = phase == E_MID ? y0 + 0.5 * dtl * s_f0[i][lane] : y0;
+        });
+        eval_fJ<M, false, FLOW>(y, pr, f, Jd);
+        if (phase == E_F0) {
+            c_rhs += 1;
+            sfor<0, N>([&](auto i) { s_f0[i][lane] = f[i]; });
+            phase = E_MID;
+        } else if (phase == E_MID) {
+            c_rhs += 1;
+            c_att += 1;
+            double e = 0.0;
+            sfor<0, N>([&](auto i) {
+                const double y0 = s_y0[i][lane], f0 = s_f0[i][lane];
+                const double yh = y[i] + 0.5 * dtl * f[i];
+                const double yf = y0 + dtl * f0;
+                const double err = 2.0 / (1.0 - 2.0) * (yh - yf);
+                const double tol = P.atol + P.rtol * fmax(fabs(yf), fabs(yh));
+                e = fmax(e, fabs(err) / tol);
+                f[i] = yh;  // keep the two-half-step state
+            });
+            if (e > 1.0) {
+                dtl *= fmax(0.8 * pow(e, -0.25), 0.25);
+                if (dtl <= kTiny) {
+                    dtl = kTiny;
+                    if (++rejects > kEEMaxTinyRejects) finish_cell(true);
+                }
+            } else {
+                t += dtl;
+                dt = e > errg ? fmin(fmax(0.8 * pow(e, -0.25), 0.25), 10.0) * dtl : 0.8 * 10.0 * dtl;
+                sfor<0, N>([&](auto i) { s_y0[i][lane] = f[i]; });
+                c_steps += 1;
+                csteps += 1;
+                if (csteps > P.max_cell_iters) {
+                    finish_cell(true);  // termination guard for every wavefront
+                } else if ((t - tn) * tn >= 0) {
+                    if (nstep > 0 && last) sub = fmin(dtTry, tn);
+                    ode_end();
+                } else {
+                    nstep++;
+                    if (!start_step()) ode_end();
+                }
+            }
+        }
+    }
+    const long long s0 = wave_sum(c_cells), s1 = wave_sum(c_steps), s2 = wave_sum(c_att),
+                    s4 = wave_sum(c_rhs), s5 = wave_max(c_maxs), s7 = wave_sum(c_fail);
+    const double dmin = wave_min(dtmin);
+    if (lane == 0) {
+        atomicAdd(&P.stats[0], (unsigned long long)s0);
+        atomicAdd(&P.stats[1], (unsigned long long)s1);
+        atomicAdd(&P.stats[2], (unsigned long long)s2);
+        atomicAdd(&P.stats[4], (unsigned long long)s4);
+        atomicMax(&P.stats[5], (unsigned long long)s5);
+        if (dmin < __builtin_huge_val())
+            atomicMin(&P.stats[6], (unsigned long long)__double_as_longlong(dmin));
+        atomicAdd(&P.stats[7], (unsigned long long)s7);
+    }
+}
+
 // ---- RHS / Jacobian evaluation at given states (chemistrySource::DyDt / jacobian) ----
 struct EvalParams {
     int64_t n;
@@ -704,7 +885,10 @@ struct LaneEntry {
 template <class M>
 struct LaneLaunch {
     static hipError_t integrate(const LaneParams &p, int grid, hipStream_t s) {
-        if (p.prec == 0) {
+        if (p.integrator == 1) {
+            if (p.flow == 0) hipLaunchKernelGGL((ee_lane_kernel<M, 0>), dim3(grid), dim3(64), 0, s, p);
+            else hipLaunchKernelGGL((ee_lane_kernel<M, 1>), dim3(grid), dim3(64), 0, s, p);
+        } else if (p.prec == 0) {
             if (p.flow == 0) hipLaunchKernelGGL((ie_lane_kernel<M, 0>), dim3(grid), dim3(64), 0, s, p);
             else hipLaunchKernelGGL((ie_lane_kernel<M, 1>), dim3(grid), dim3(64), 0, s, p);
         } else {

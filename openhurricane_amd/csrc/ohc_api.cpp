@@ -298,6 +298,7 @@ int chem_integrate(ohc_ctx *c, int mode, int64_t n, const double *T, double *Tio
     P.stats = c->d_ctl + 1;
     P.exact_dkp = o.jacobian;
     P.prec = o.precision;
+    P.integrator = o.integrator;
     hipLaunchKernelGGL(reset_ctl, dim3(1), dim3(64), 0, c->stream, c->d_ctl);
     HIPCHK(c, hipGetLastError());
     if (P.n > 0) {
@@ -471,6 +472,10 @@ int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_
     if (!c->lane && o.precision == 1)
         return fail(nullptr, OHC_ERR_UNSUPPORTED,
                     "the FP32 split precision mode needs a specialised kernel for this mechanism");
+    if (o.integrator != 0 && o.integrator != 1) return fail(nullptr, OHC_ERR_ARG, "integrator must be 0 or 1");
+    if (o.integrator == 1 && (!c->lane || o.precision != 0))
+        return fail(nullptr, OHC_ERR_UNSUPPORTED,
+                    "the explicit Euler comparison integrator runs on specialised FP64 kernels only");
     if (!c->lane && o.kernel == 1)
         return fail(nullptr, OHC_ERR_UNSUPPORTED,
                     "no specialised kernel compiled for this mechanism; compiled: " +

@@ -990,6 +990,90 @@ static int cell_integrate(ie_ws *s, double T, double p, double rho, double leftT
     return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* Reference GPU integrator (comparison mode): explicit Euler with a      */
+/* step-doubling error estimate -- EulerCUDA::{solve, adaptiveSolve,     */
+/* solve(t0,tn,dt0,...)} (CS/src/EulerCUDA.cu:34-62, :130-224),          */
+/* ODEsSolverCUDA::maxError (CS/src/ODEsSolverCUDA.cu:34-48) and the      */
+/* per-cell loop of solveChemODEs (CS/src/CUDAChemistrySourceODEs.cu:     */
+/* 588-640).  The state is y = [c_1..c_ns, T] as in the implicit path.    */
+/* ------------------------------------------------------------------ */
+#define EE_MAX_TINY_REJECTS 1000 /* rejected attempts at dt = tiny before a cell fails */
+
+/* one attempt of size dt from y0 with f0 = f(y0): y <- two half steps,
+ * returns max |2 (y_full - y_half)| / (atol + rtol max(|y_full|, |y_half|)) */
+static double ee_attempt(ie_ws *s, const double *y0, const double *f0, double dt, double *y) {
+    const int n = s->n;
+    double ym[NMAX], fm[NMAX];
+    for (int j = 0; j < n; ++j) ym[j] = y0[j] + 0.5 * dt * f0[j];
+    orc_dydt(s->m, s->o->flow_type, s->pr, ym, fm);
+    s->rhs++;
+    double e = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const double yh = ym[j] + 0.5 * dt * fm[j];
+        const double yf = y0[j] + dt * f0[j];
+        const double err = 2.0 / (1.0 - 2.0) * (yh - yf);
+        const double tol = s->o->atol + s->o->rtol * dmax(fabs(yf), fabs(yh));
+        e = dmax(e, fabs(err) / tol);
+        y[j] = yh;
+    }
+    return e;
+}
+
+/* EulerCUDA::adaptiveSolve: y advanced in place, *dt0 the next step */
+static int ee_adaptive(ie_ws *s, double *t, double *dt0, double *y) {
+    const int n = s->n;
+    double f0[NMAX], y1[NMAX];
+    orc_dydt(s->m, s->o->flow_type, s->pr, y, f0);
+    s->rhs++;
+    double dt = *dt0, err;
+    int tiny_rejects = 0;
+    for (;;) {
+        s->attempts++;
+        err = ee_attempt(s, y, f0, dt, y1);
+        if (!(err > 1.0)) break;
+        dt *= dmax(0.8 * pow(err, -0.25), 0.25);
+        if (dt <= TINY) {
+            dt = TINY;
+            if (++tiny_rejects > EE_MAX_TINY_REJECTS) return OHC_ERR_NUMERIC;
+        }
+    }
+    *t += dt;
+    if (err > pow(10.0 / 0.8, -1.0 / 0.25))
+        *dt0 = dmin(dmax(0.8 * pow(err, -0.25), 0.25), 10.0) * dt;
+    else
+        *dt0 = 0.8 * 10.0 * dt;
+    memcpy(y, y1, sizeof(double) * n);
+    s->steps++;
+    s->cell_steps++;
+    return 0;
+}
+
+/* EulerCUDA::solve(odes, t0, tn, dt0, ...) */
+static int ee_solve(ie_ws *s, double *tn, double *dt0, double *y) {
+    const double t0 = 0.0;
+    double t = t0, dt = *dt0, dtReached = t0;
+    int last = 0;
+    for (int n = 0; n < s->o->max_steps; ++n) {
+        const double dtTry0 = dt;
+        if ((t + dt - *tn) * (t + dt - t0) > 0) {
+            dt = *tn - t;
+            last = 1;
+        } else if (t + dt == *tn) {
+            last = 1;
+        }
+        int rc = ee_adaptive(s, &t, &dt, y);
+        if (rc) return rc;
+        dtReached = t;
+        if ((t - *tn) * (*tn - t0) >= 0) {
+            if (n > 0 && last) *dt0 = dmin(dtTry0, *tn);
+            return 0;
+        }
+    }
+    *tn = dtReached;
+    return 0;
+}
+
 typedef struct {
     const orc_mech *m;
     const ohc_opts *o;
@@ -1021,6 +1105,46 @@ static void *solve_range(void *arg) {
         const double dtc = j->dt[cell] * j->dtFactor;
         double dtI = j->subDt[cell] > 0 ? j->subDt[cell] : dtc; /* chemistrySource.inl:86-95 */
         if (j->cell_steps) j->cell_steps[cell] = 0;
+        if (j->o->integrator == 1) {
+            /* CUDAChemistrySource::solve / solveUpdate: every cell, dt0 = subDt */
+            const double pi = j->p[cell], rhoi = j->rho[cell];
+            const double *Ysrc = j->update ? j->Y : j->Yin;
+            double y[NMAX];
+            for (int k = 0; k < ns; ++k) {
+                c0[k] = rhoi * Ysrc[k * j->ldY + cell] / m->W[k];
+                y[k] = c0[k];
+            }
+            y[ns] = Ti;
+            s->pr = j->o->flow_type == 0 ? pi : rhoi;
+            s->cell_steps = 0;
+            double left = dtc, sub = dtI;
+            int rc = 0;
+            while (left > 0.0 && !rc) {
+                double dtn = left;
+                rc = ee_solve(s, &dtn, &sub, y);
+                left -= dtn;
+            }
+            if (rc) {
+                j->rc = rc;
+                break;
+            }
+            j->st.cells++;
+            j->st.dt_min = dmin(j->st.dt_min, sub);
+            if (s->cell_steps > j->st.max_cell_steps) j->st.max_cell_steps = s->cell_steps;
+            if (j->cell_steps) j->cell_steps[cell] = (int)s->cell_steps;
+            j->subDt[cell] = sub;
+            if (!j->update) {
+                for (int k = 0; k < ns; ++k)
+                    j->R[k * j->ldR + cell] = (y[k] - c0[k]) / dtc * m->W[k];
+            } else {
+                /* device ciToYi (CUDAChemistrySourceODEs.cu:545-586): Y = c W / sum c W */
+                double r = 0.0;
+                for (int k = 0; k < ns; ++k) r += y[k] * m->W[k];
+                for (int k = 0; k < ns; ++k) j->Y[k * j->ldY + cell] = y[k] * m->W[k] / r;
+                j->T[cell] = y[ns];
+            }
+            continue;
+        }
         if (Ti < j->o->T_threshold) {
             j->subDt[cell] = dtc;
             if (!j->update)

@@ -211,3 +211,25 @@ def test_wilke_mixture_rules(h2mech, h2oracle):
                      np.sqrt(8 * (1 + w[a] / w[b])) for b in range(2)] for a in range(2)])
     mix = sum(x[a] * m[a] / (phi[a] @ x) for a in range(2))
     assert mu == pytest.approx(mix, rel=1e-13)
+
+
+def test_explicit_euler_restatement_converges(h2mech, h2oracle):
+    """The reference GPU integrator's restatement (explicit Euler, step
+    doubling) approaches a tight implicit solution as rtol tightens."""
+    from openhurricane_amd import _lib
+    from openhurricane_amd.sampler import h2air_states
+
+    n = 8
+    st = h2air_states(h2mech.species, h2mech.tables()["W"], n, seed=11)
+    dt = np.full(n, 1e-6)
+    ref = _lib.default_opts(atol=1e-16, rtol=1e-9, atol_inc=1e-16, rtol_inc=1e-9)
+    Tr, Yr, _, _ = h2oracle.solve_update(ref, st["T"], st["p"], st["rho"], st["Y"], dt, 1.0,
+                                         subDt=np.full(n, 1e-13))
+    errs = []
+    for rtol in (1e-3, 1e-5, 1e-7):
+        o = _lib.default_opts(atol=1e-14, rtol=rtol, integrator=1)
+        T, Y, _, s = h2oracle.solve_update(o, st["T"], st["p"], st["rho"], st["Y"], dt, 1.0)
+        assert s["newton_iters"] == 0 and s["rhs_evals"] == s["attempts"] + s["steps"]
+        np.testing.assert_allclose(Y.sum(0), 1.0, rtol=1e-13)
+        errs.append(np.max(np.abs(Y - Yr) / np.maximum(Yr, 1e-3)))
+    assert errs[0] > errs[1] > errs[2], errs
