@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--workload", default="h2air", choices=["h2air", "syn32", "syn54"],
                     help="h2air: 9-species H2-air (default); syn32 / syn54: synthetic "
                          "32-species/150-reaction or 54/270 mechanisms on the generic kernel")
+    ap.add_argument("--no-split", action="store_true",
+                    help="skip the FP32 split-precision comparison")
     ap.add_argument("--jacobian", type=int, default=0,
                     help="0: reference Jacobian (its dKp/dT sign convention); 1: exact")
     args = ap.parse_args()
@@ -159,6 +161,7 @@ def main():
             "kernel": "ie_lane_kernel<Mech_h2air>", "kernel_ms": kern_ms,
             "flops_per_launch": fl, "newton_iters": stats["newton_iters"],
             "steps": stats["steps"], "flops_per_newton": integrator_flops(tab)["F_newton"]}
+    split = None if args.no_split else split_precision(args, mech, g, T0, Y0, dt, stream)
     cpu = None
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
@@ -174,9 +177,62 @@ def main():
                                "ImplicitEuler (rtol 1e-6, atol 1e-10) + mu/kappa/D_k",
                    "cells": n, "mechanism": "h2air", "integrator": "ImplicitEuler",
                    "parallelism": "single GPU"},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "split_precision": split,
     }
     print(json.dumps(line), flush=True)
+
+
+def split_precision(args, mech, g, T0, Y0, dt, stream):
+    """The same flow step with FP32 Gibbs / equilibrium / rate constants and
+    FP32 transport (FP64 LU, T(h), sums): cells/s and the per-output relative
+    error against the FP64 path on every cell (the FP64 device path equals the
+    FP64 oracle to 1e-12)."""
+    import torch
+
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource
+
+    kw = dict(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6, device_pointers=1,
+              schedule=args.schedule, jacobian=args.jacobian)
+    res = {}
+    for prec in (0, 1):
+        c = ChemistrySource(mech, device=0, opts=_lib.default_opts(precision=prec, **kw))
+        c.set_stream(stream.cuda_stream)
+        sub = torch.zeros_like(T0)
+        times = []
+        for _ in range(2 + args.steps):
+            T, Y = T0.clone(), Y0.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            c.solve_update(T, g["p"], g["rho"], Y, dt, 1.0, subDt=sub, stats=False)
+            props = c.mix_props(T, g["p"], Y, mu=True, kappa=True, Dk=True, cp=False, ha=False)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        same = c.mix_props(T0, g["p"], Y0, mu=True, kappa=True, Dk=True, cp=False, ha=False)
+        res[prec] = (T, Y, same, float(np.mean(times[2:])))
+    T64, Y64, p64, t64 = res[0]
+    T32, Y32, p32, t32 = res[1]
+    eT = ((T32 - T64).abs() / T64)
+    major = Y64 >= 1e-3
+    eY = torch.where(major, (Y32 - Y64).abs() / Y64.clamp_min(1e-300), torch.zeros_like(Y64))
+    eYm = eY[major]
+    out = {"value": T0.shape[0] / t32, "unit": "cells/s", "fp64_value": T0.shape[0] / t64,
+           "max_rel_err": {}, "mean_rel_err": {}}
+    out["max_rel_err"]["T"] = float(eT.max())
+    out["mean_rel_err"]["T"] = float(eT.mean())
+    out["median_rel_err_T"] = float(eT.median())
+    out["max_rel_err"]["Y_major"] = float(eYm.max())
+    out["mean_rel_err"]["Y_major"] = float(eYm.mean())
+    for k in ("mu", "kappa"):
+        e = (p32[k] - p64[k]).abs() / p64[k].abs()
+        out["max_rel_err"][k] = float(e.max())
+        out["mean_rel_err"][k] = float(e.mean())
+    eD = (p32["Dk"] - p64["Dk"]).abs() / p64["Dk"].abs().amax(0).clamp_min(1e-300)
+    out["max_rel_err"]["Dk"] = float(eD.max())
+    out["mean_rel_err"]["Dk"] = float(eD.mean())
+    out["note"] = ("errors of the state after one flow step (T, Y with Y >= 1e-3) and of "
+                   "mu/kappa/D_k at the same input state, FP32 split vs FP64")
+    return out
 
 
 SYNTH = {"syn32": (32, 150, 1, 1 << 22), "syn54": (54, 270, 2, 1 << 23)}

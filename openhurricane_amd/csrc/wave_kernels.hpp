@@ -596,9 +596,10 @@ __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, 
             const bool el = step < 0;
             const double l = el ? a[k] * rp : 0.0;
             if (el) a[k] = l;
+            // columns j >= n hold zeros in every row and stay zero, so the
+            // update needs no per-column predicate
 #pragma unroll
-            for (int j = k + 1; j < N; ++j)
-                if (j < n) a[j] = fma(-l, rdlane(a[j], p), a[j]);
+            for (int j = k + 1; j < N; ++j) a[j] = fma(-l, rdlane(a[j], p), a[j]);
             b = fma(-l, rdlane(b, p), b);
         }
     }
