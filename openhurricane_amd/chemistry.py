@@ -183,6 +183,13 @@ class ChemistrySource:
         h = C.c_void_p()
         check(lib().ohc_ctx_create(mech._h, C.byref(self.opts), device, C.byref(h)))
         self._h = h
+        if self.device_pointers:
+            import torch
+
+            # torch produces and consumes the tensors on its current stream;
+            # the library's kernels run on that same stream so that both
+            # stay ordered without extra synchronisation
+            self.set_stream(torch.cuda.current_stream(device).cuda_stream)
 
     def __del__(self):
         try:

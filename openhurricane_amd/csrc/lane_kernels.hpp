@@ -460,19 +460,44 @@ __global__ __launch_bounds__(64) void ee_lane_kernel(const LaneParams P) {
     constexpr int NS = M::NS, N = NS + 1;
     enum { E_NEED, E_F0, E_MID, E_DONE };
     const int lane = threadIdx.x;
+    // Every loop-carried lane value except the phase lives in LDS (one slot
+    // per lane): the loop body then holds only y, f and transients in
+    // registers.
     __shared__ double s_y0[N][64], s_f0[N][64];
-    int phase = E_NEED, last = 0, nstep = 0, rejects = 0;
-    long long cell = 0, csteps = 0;
-    double pr = 0.0, t = 0.0, dt = 0.0, dtl = 0.0, dtTry = 0.0, tn = 0.0, left = 0.0, sub = 0.0,
-           dtc = 0.0;
-    long long c_cells = 0, c_steps = 0, c_att = 0, c_rhs = 0, c_maxs = 0, c_fail = 0;
-    double dtmin = __builtin_huge_val();
-    const double errg = pow(10.0 / 0.8, -1.0 / 0.25);
+    enum { C_PR, C_T, C_DT, C_DTL, C_TRY, C_TN, C_LEFT, C_SUB, C_DTC, C_DTMIN, C_NCTL };
+    enum { S_CELLS, S_STEPS, S_ATT, S_RHS, S_MAXS, S_FAIL, S_CELL, S_CSTEPS, S_NSTEP, S_LAST,
+           S_REJ, S_NST };
+    __shared__ double s_c[C_NCTL][64];
+    __shared__ long long s_n[S_NST][64];
+    enum { Q_ATOL, Q_RTOL, Q_DTF, Q_ERRG, Q_NPAR };
+    __shared__ double s_par[Q_NPAR];
+    if (lane == 0) {
+        s_par[Q_ATOL] = P.atol;
+        s_par[Q_RTOL] = P.rtol;
+        s_par[Q_DTF] = P.dtFactor;
+        s_par[Q_ERRG] = pow(10.0 / 0.8, -1.0 / 0.25);
+    }
+#define CT(i) s_c[i][lane]
+#define CNT(i) s_n[i][lane]
+#pragma unroll
+    for (int i = 0; i < C_NCTL; ++i) CT(i) = 0.0;
+#pragma unroll
+    for (int i = 0; i < S_NST; ++i) CNT(i) = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        s_y0[i][lane] = 0.0;
+        s_f0[i][lane] = 0.0;
+    }
+    s_y0[NS][lane] = 1000.0;
+    CT(C_DTMIN) = __builtin_huge_val();
+    __syncthreads();
+    int phase = E_NEED;
 
-    auto finish_cell = [&](bool failed) {
-        P.subDt[cell] = sub;
+    auto finish_cell = [&](bool failed) __attribute__((always_inline)) {
+        const long long cell = CNT(S_CELL);
+        P.subDt[cell] = CT(C_SUB);
         if (P.mode == 0) {
-            const double rho = P.rho[cell], rdt = 1.0 / dtc;
+            const double rho = P.rho[cell], rdt = 1.0 / CT(C_DTC);
             sfor<0, NS>([&](auto k) {
                 const double c0 = rho * P.Y[k * P.ldY + cell] / M::W[k];
                 P.R[k * P.ldR + cell] = (s_y0[k][lane] - c0) * rdt * M::W[k];
@@ -485,43 +510,48 @@ __global__ __launch_bounds__(64) void ee_lane_kernel(const LaneParams P) {
             sfor<0, NS>([&](auto k) { P.Yout[k * P.ldY + cell] = s_y0[k][lane] * M::W[k] * rr; });
             P.Tout[cell] = s_y0[NS][lane];
         }
-        if (P.cost) P.cost[cell] = (int)(csteps < 0x7fffffffll ? csteps : 0x7fffffffll);
-        c_cells += 1;
-        c_fail += failed ? 1 : 0;
-        c_maxs = csteps > c_maxs ? csteps : c_maxs;
-        dtmin = fmin(dtmin, sub);
+        const long long cs = CNT(S_CSTEPS);
+        if (P.cost) P.cost[cell] = (int)(cs < 0x7fffffffll ? cs : 0x7fffffffll);
+        CNT(S_CELLS) += 1;
+        if (failed) CNT(S_FAIL) += 1;
+        if (cs > CNT(S_MAXS)) CNT(S_MAXS) = cs;
+        CT(C_DTMIN) = fmin(CT(C_DTMIN), CT(C_SUB));
         phase = E_NEED;
     };
     // EulerCUDA::solve loop head; false when the step budget is exhausted
-    auto start_step = [&]() -> bool {
-        if (nstep >= P.max_steps) {
-            tn = t;
+    auto start_step = [&]() __attribute__((always_inline)) -> bool {
+        const double t = CT(C_T), tn = CT(C_TN);
+        double dt = CT(C_DT);
+        if (CNT(S_NSTEP) >= P.max_steps) {
+            CT(C_TN) = t;
             return false;
         }
-        dtTry = dt;
+        CT(C_TRY) = dt;
         if ((t + dt - tn) * (t + dt) > 0) {
             dt = tn - t;
-            last = 1;
+            CNT(S_LAST) = 1;
         } else if (t + dt == tn) {
-            last = 1;
+            CNT(S_LAST) = 1;
         }
-        dtl = dt;
-        rejects = 0;
+        CT(C_DT) = dt;
+        CT(C_DTL) = dt;
+        CNT(S_REJ) = 0;
         phase = E_F0;
         return true;
     };
-    auto ode_end = [&]() {
-        left -= tn;
+    // end of one EulerCUDA::solve call over [0, tn]: the cell either has time
+    // left (the budget ran out first) or is finished
+    auto ode_end = [&]() __attribute__((always_inline)) {
+        const double left = CT(C_LEFT) - CT(C_TN);
+        CT(C_LEFT) = left;
         if (left > 0.0) {
-            tn = left;
-            t = 0.0;
-            dt = sub;
-            last = 0;
-            nstep = 0;
-            if (!start_step()) {
-                // no step possible: tn = t = 0 would not advance; give up on the cell
-                finish_cell(true);
-            }
+            CT(C_TN) = left;
+            CT(C_T) = 0.0;
+            CT(C_DT) = CT(C_SUB);
+            CNT(S_LAST) = 0;
+            CNT(S_NSTEP) = 0;
+            // no step possible (a zero budget): give up on the cell
+            if (!start_step()) finish_cell(true);
         } else {
             finish_cell(false);
         }
@@ -541,76 +571,90 @@ __global__ __launch_bounds__(64) void ee_lane_kernel(const LaneParams P) {
                 if (mine >= (unsigned long long)P.n) {
                     phase = E_DONE;
                 } else {
-                    cell = P.order ? (long long)P.order[mine] : (long long)mine;
+                    const long long cell = P.order ? (long long)P.order[mine] : (long long)mine;
+                    CNT(S_CELL) = cell;
                     const double rho = P.rho[cell];
-                    pr = FLOW == 0 ? P.p[cell] : rho;
-                    dtc = P.dt[cell] * P.dtFactor;
+                    CT(C_PR) = FLOW == 0 ? P.p[cell] : rho;
+                    const double dtc = P.dt[cell] * s_par[Q_DTF];
                     const double s0 = P.subDt[cell];
-                    sub = s0 > 0.0 ? s0 : dtc;
+                    const double sub = s0 > 0.0 ? s0 : dtc;
+                    CT(C_DTC) = dtc;
+                    CT(C_SUB) = sub;
                     sfor<0, NS>([&](auto k) { s_y0[k][lane] = rho * P.Y[k * P.ldY + cell] / M::W[k]; });
                     s_y0[NS][lane] = P.T[cell];
-                    csteps = 0;
-                    left = dtc;
-                    tn = left;
-                    t = 0.0;
-                    dt = sub;
-                    last = 0;
-                    nstep = 0;
+                    CNT(S_CSTEPS) = 0;
+                    CT(C_LEFT) = dtc;
+                    CT(C_TN) = dtc;
+                    CT(C_T) = 0.0;
+                    CT(C_DT) = sub;
+                    CNT(S_LAST) = 0;
+                    CNT(S_NSTEP) = 0;
                     if (!start_step()) finish_cell(true);
                 }
             }
         }
         if (!__any(phase == E_F0 || phase == E_MID)) break;
+        // one f evaluation per lane: at y0 (E_F0) or at the half-step point
         double y[N], f[N], Jd[N][N];
-        sfor<0, N>([&](auto i) {
-            const double y0 = s_y0[i][lane];
-            y[i] = phase == E_MID ? y0 + 0.5 * dtl * s_f0[i][lane] : y0;
-        });
-        eval_fJ<M, false, FLOW>(y, pr, f, Jd);
+        {
+            const double hh = phase == E_MID ? 0.5 * CT(C_DTL) : 0.0;
+            sfor<0, N>([&](auto i) { y[i] = s_y0[i][lane] + hh * s_f0[i][lane]; });
+        }
+        eval_fJ<M, false, FLOW>(y, CT(C_PR), f, Jd);
         if (phase == E_F0) {
-            c_rhs += 1;
+            CNT(S_RHS) += 1;
             sfor<0, N>([&](auto i) { s_f0[i][lane] = f[i]; });
             phase = E_MID;
         } else if (phase == E_MID) {
-            c_rhs += 1;
-            c_att += 1;
+            CNT(S_RHS) += 1;
+            CNT(S_ATT) += 1;
+            const double dtl = CT(C_DTL), atol = s_par[Q_ATOL], rtol = s_par[Q_RTOL];
             double e = 0.0;
             sfor<0, N>([&](auto i) {
                 const double y0 = s_y0[i][lane], f0 = s_f0[i][lane];
                 const double yh = y[i] + 0.5 * dtl * f[i];
                 const double yf = y0 + dtl * f0;
                 const double err = 2.0 / (1.0 - 2.0) * (yh - yf);
-                const double tol = P.atol + P.rtol * fmax(fabs(yf), fabs(yh));
+                const double tol = atol + rtol * fmax(fabs(yf), fabs(yh));
                 e = fmax(e, fabs(err) / tol);
                 f[i] = yh;  // keep the two-half-step state
             });
             if (e > 1.0) {
-                dtl *= fmax(0.8 * pow(e, -0.25), 0.25);
-                if (dtl <= kTiny) {
-                    dtl = kTiny;
-                    if (++rejects > kEEMaxTinyRejects) finish_cell(true);
+                double d = dtl * fmax(0.8 * pow(e, -0.25), 0.25);
+                if (d <= kTiny) {
+                    d = kTiny;
+                    CNT(S_REJ) += 1;
                 }
+                CT(C_DTL) = d;
+                // the attempt at y0 is repeated with the smaller step (f0 kept)
+                if (CNT(S_REJ) > kEEMaxTinyRejects) finish_cell(true);
             } else {
-                t += dtl;
-                dt = e > errg ? fmin(fmax(0.8 * pow(e, -0.25), 0.25), 10.0) * dtl : 0.8 * 10.0 * dtl;
+                const double t = CT(C_T) + dtl, tn = CT(C_TN);
+                CT(C_T) = t;
+                CT(C_DT) = e > s_par[Q_ERRG] ? fmin(fmax(0.8 * pow(e, -0.25), 0.25), 10.0) * dtl
+                                              : 0.8 * 10.0 * dtl;
                 sfor<0, N>([&](auto i) { s_y0[i][lane] = f[i]; });
-                c_steps += 1;
-                csteps += 1;
-                if (csteps > P.max_cell_iters) {
+                CNT(S_STEPS) += 1;
+                const long long cs = CNT(S_CSTEPS) + 1;
+                CNT(S_CSTEPS) = cs;
+                if (cs > P.max_cell_iters) {
                     finish_cell(true);  // termination guard for every wavefront
                 } else if ((t - tn) * tn >= 0) {
-                    if (nstep > 0 && last) sub = fmin(dtTry, tn);
+                    if (CNT(S_NSTEP) > 0 && CNT(S_LAST)) CT(C_SUB) = fmin(CT(C_TRY), tn);
                     ode_end();
                 } else {
-                    nstep++;
+                    CNT(S_NSTEP) += 1;
                     if (!start_step()) ode_end();
                 }
             }
         }
     }
-    const long long s0 = wave_sum(c_cells), s1 = wave_sum(c_steps), s2 = wave_sum(c_att),
-                    s4 = wave_sum(c_rhs), s5 = wave_max(c_maxs), s7 = wave_sum(c_fail);
-    const double dmin = wave_min(dtmin);
+    const long long s0 = wave_sum(CNT(S_CELLS)), s1 = wave_sum(CNT(S_STEPS)),
+                    s2 = wave_sum(CNT(S_ATT)), s4 = wave_sum(CNT(S_RHS)),
+                    s5 = wave_max(CNT(S_MAXS)), s7 = wave_sum(CNT(S_FAIL));
+    const double dmin = wave_min(CT(C_DTMIN));
+#undef CT
+#undef CNT
     if (lane == 0) {
         atomicAdd(&P.stats[0], (unsigned long long)s0);
         atomicAdd(&P.stats[1], (unsigned long long)s1);
