@@ -277,8 +277,23 @@ int ohc_surrogate_set_params(ohc_surrogate *s, const float *params);
 int ohc_surrogate_set_norm(ohc_surrogate *s, const float *in_mean, const float *in_std,
                            const float *in_lo, const float *in_hi, const float *out_mean,
                            const float *out_std, const float *out_weight, double dt_train);
+/* Target transform (host array of ns+1 scales, NULL: none): output k is
+ * trained on z = sign(d) ln(1 + |d| / delta_k) of the increment d, and the
+ * inference inverts it, so that small increments are resolved as finely as
+ * large ones.  Set before computing the training targets. */
+int ohc_surrogate_set_target_scale(ohc_surrogate *s, const float *delta);
+/* Inference precision: 0 BF16 (default); 1 FP8 E4M3 weights (per-row scale)
+ * and activations for the hidden-to-hidden layers on the f8f6f4 matrix
+ * instructions, BF16 first and last layers.  Training stays BF16. */
+int ohc_surrogate_set_precision(ohc_surrogate *s, int32_t precision);
+/* Gate on a second network of the same context (NULL: none): a cell passes
+ * only when both networks agree on dT within tol * T and on the ln Y
+ * increment of every species with Y >= 1e-3 within tol; it then takes the
+ * mean of the two increments.  The peer must outlive the calls. */
+int ohc_surrogate_set_peer(ohc_surrogate *s, ohc_surrogate *peer, double tol);
 /* Raw features X [n][ns+2] of states (T0, p, Y0) and, when T1/Y1 are given,
- * raw targets Yt [n][ns+1] of the pairs (T0, Y0) -> (T1, Y1). */
+ * targets Yt [n][ns+1] of the pairs (T0, Y0) -> (T1, Y1): the increments, or
+ * their transform when target scales are set. */
 int ohc_surrogate_features(ohc_surrogate *s, int64_t n, const double *T0, const double *p,
                            const double *Y0, int64_t ldY, const double *T1, const double *Y1,
                            float *X, float *Yt);
@@ -291,13 +306,15 @@ int ohc_surrogate_train_step(ohc_surrogate *s, int64_t n, const float *X, const 
  * layout) for a batch -- the backward kernels without the update. */
 int ohc_surrogate_gradient(ohc_surrogate *s, int64_t n, const float *X, const float *Yt,
                            float *grad, float *loss);
-/* Predicted raw increments dout [n][ns+1] at states (T, p, Y). */
+/* Predicted increments dout [n][ns+1] (dT, d ln Y; the target transform
+ * inverted) at states (T, p, Y). */
 int ohc_surrogate_predict(ohc_surrogate *s, int64_t n, const double *T, const double *p,
                           const double *Y, int64_t ldY, float *dout);
 /* solveUpdate with the surrogate: every cell is advanced by the network unless
  * its gate fails -- time step differs from the training step, a feature lies
- * outside the accepted box, the new T leaves [T_low, T_high], or the relative
- * change of any element's mass exceeds gate_tol -- and the failed cells
+ * outside the accepted box, the new T leaves [T_low, T_high], the relative
+ * change of any element's mass exceeds gate_tol, or the peer network (if
+ * set) disagrees -- and the failed cells
  * (count in *n_fallback) are integrated by the plain integrator, whose
  * counters go to stats. */
 int ohc_chem_solve_update_surrogate(ohc_ctx *ctx, ohc_surrogate *s, int64_t n, double *T,

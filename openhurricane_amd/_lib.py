@@ -108,6 +108,9 @@ _SIGS += [
     ("ohc_chem_solve_update_surrogate", C.c_int, [_V, _V, C.c_int64, _V, _V, _V, _V, C.c_int64,
                                                   _V, C.c_double, _V, C.c_double,
                                                   C.POINTER(C.c_int64), C.POINTER(Stats)]),
+    ("ohc_surrogate_set_target_scale", C.c_int, [_V, _V]),
+    ("ohc_surrogate_set_precision", C.c_int, [_V, C.c_int32]),
+    ("ohc_surrogate_set_peer", C.c_int, [_V, _V, C.c_double]),
     ("ohc_surrogate_save", C.c_int, [_V, C.c_char_p]),
     ("ohc_surrogate_load", C.c_int, [_V, C.c_char_p]),
 ]

@@ -265,6 +265,44 @@ namespace ohcm {
 
 constexpr double kYFloor = 1e-20;  // mass-fraction floor of the log features
 
+typedef __attribute__((__vector_size__(8 * sizeof(int)))) int i32x8;
+
+// OCP FP8 E4M3 (bias 7, no infinities, 0x7f NaN, largest finite 448): round to
+// nearest even, saturating.
+__host__ __device__ inline uint8_t f2e4m3(float x) {
+    const uint8_t sign = x < 0.0f ? 0x80 : 0x00;
+    const float a = fabsf(x);
+    if (!(a == a)) return 0x7f;
+    if (a >= 448.0f) return sign | 0x7e;
+    if (a < 0.015625f) {  // subnormal: m * 2^-9
+        const int m = (int)rintf(a * 512.0f);
+        return sign | (uint8_t)m;  // m == 8 is the smallest normal, 0x08
+    }
+    int ex;
+    const float fr = frexpf(a, &ex);  // a = fr 2^ex, fr in [0.5, 1)
+    int e = ex - 1;
+    int m = (int)rintf((2.0f * fr - 1.0f) * 8.0f);
+    if (m == 8) {
+        m = 0;
+        ++e;
+    }
+    const int be = e + 7;
+    if (be > 15 || (be == 15 && m == 7)) return sign | 0x7e;
+    return sign | (uint8_t)((be << 3) | m);
+}
+// The same rounding with the hardware conversion (v_cvt_pk_fp8_f32 is OCP
+// E4M3 with round to nearest even on this target, NaN above 448, hence the
+// clamp; a negative value that rounds to zero gives +0 instead of -0).
+__device__ __forceinline__ uint8_t f2e4m3_hw(float x) {
+    const float c = fminf(fmaxf(x, -448.0f), 448.0f);
+    return (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(c, c, 0, false) & 0xff);
+}
+__host__ __device__ inline float e4m32f(uint8_t b) {
+    const int e = (b >> 3) & 15, m = b & 7;
+    const float v = e == 0 ? ldexpf((float)m, -9) : ldexpf(1.0f + 0.125f * (float)m, e - 7);
+    return (b & 0x80) ? -v : v;
+}
+
 struct InferArgs {
     int64_t n;
     int ns, ne;                 // species, elements
@@ -277,22 +315,39 @@ struct InferArgs {
     const float *b;             // biases, layer l at boff[l] (padded rows zero)
     int woff[8], boff[8];
     int depth;                  // hidden layers
+    const uint8_t *W8;          // FP8 copies of the hidden-to-hidden layers l = 1..depth-1
+    const float *s8;            // their per-row scales
+    int w8off[8], s8off[8];
     const float *in_mean, *in_rstd, *zlo, *zhi;  // input normalisation and OOD box
     const float *out_mean, *out_std;
+    const float *delta;         // symmetric-log target scales (nullptr: plain increments)
+    const float *peer;          // [n][nout] increments of a second network (nullptr: none)
+    double peer_tol;            // largest accepted disagreement with the peer
     const double *elemW;        // [ns][ne]: atoms of element e in species k / W_k
     double gate_tol;            // max relative element-mass change
     double T_low, T_high;
     int *fail_list;             // indices of cells left to the integrator
     unsigned int *fail_count;
-    int mode;                   // 0: advance and gate; 1: write raw outputs to out32 only
-    float *out32;               // mode 1: [n][nout] denormalised increments
+    int mode;                   // 0: advance and gate; 1: write increments to out32 only
+    float *out32;               // mode 1: [n][nout] increments (dT, dln Y)
 };
 
+// network output k (denormalised) -> increment: identity, or the inverse of
+// z = sign(d) ln(1 + |d| / delta_k)
+__device__ __forceinline__ float out_increment(const InferArgs &a, const float *o, int k) {
+    const float z = o[k] * a.out_std[k] + a.out_mean[k];
+    if (!a.delta) return z;
+    return copysignf(expm1f(fabsf(z)), z) * a.delta[k];
+}
+
 // Hidden layer: out[TM][W] = gelu(in[TM][K] W^T + b), 4 wavefronts each
-// producing TM rows x W/4 columns with 32x32x16 BF16 MFMA.
-template <int TM, int WD, int K, int LDI, int LDO>
-__device__ __forceinline__ void mlp_layer(const bf16_t *sin, bf16_t *sout, const bf16_t *Wl,
-                                          const float *bl) {
+// producing TM rows x W/4 columns.  IN8: FP8 activations and weights on the
+// 32x32x64 f8f6f4 MFMA (unit block scales; per-row weight scale applied in
+// the epilogue), else BF16 on 32x32x16.  OUT8: the epilogue writes FP8.
+// Strides LDI / LDO count elements of the buffer's type.
+template <int TM, int WD, int K, int LDI, int LDO, bool IN8, bool OUT8>
+__device__ __forceinline__ void mlp_layer(const void *sin_, void *sout_, const void *Wl_,
+                                          const float *ws, const float *bl) {
     constexpr int RT = TM / 32, CT = WD / 128;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     f32x16 acc[RT][CT];
@@ -300,43 +355,73 @@ __device__ __forceinline__ void mlp_layer(const bf16_t *sin, bf16_t *sout, const
     for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int c = 0; c < CT; ++c) acc[r][c] = f32x16{};
-    const int i = lane & 31, kh = 8 * (lane >> 5);
+    const int i = lane & 31, h = lane >> 5;
+    if constexpr (IN8) {
+        const uint8_t *sin = (const uint8_t *)sin_, *Wl = (const uint8_t *)Wl_;
 #pragma unroll
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        bf16x8 a[RT], b[CT];
+        for (int k0 = 0; k0 < K; k0 += 64) {
+            i32x8 a[RT], b[CT];
 #pragma unroll
-        for (int r = 0; r < RT; ++r) a[r] = *(const bf16x8 *)(sin + (r * 32 + i) * LDI + k0 + kh);
+            for (int r = 0; r < RT; ++r) a[r] = *(const i32x8 *)(sin + (r * 32 + i) * LDI + k0 + 32 * h);
 #pragma unroll
-        for (int c = 0; c < CT; ++c) {
-            const int nn = wave * (CT * 32) + c * 32 + i;
-            b[c] = *(const bf16x8 *)(Wl + (int64_t)nn * K + k0 + kh);
+            for (int c = 0; c < CT; ++c) {
+                const int nn = wave * (CT * 32) + c * 32 + i;
+                b[c] = *(const i32x8 *)(Wl + (int64_t)nn * K + k0 + 32 * h);
+            }
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int c = 0; c < CT; ++c)
+                    acc[r][c] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                        a[r], b[c], acc[r][c], 0, 0, 0, 127, 0, 127);
         }
+    } else {
+        const bf16_t *sin = (const bf16_t *)sin_, *Wl = (const bf16_t *)Wl_;
+        const int kh = 8 * h;
 #pragma unroll
-        for (int r = 0; r < RT; ++r)
+        for (int k0 = 0; k0 < K; k0 += 16) {
+            bf16x8 a[RT], b[CT];
 #pragma unroll
-            for (int c = 0; c < CT; ++c)
-                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[r], b[c], acc[r][c], 0, 0, 0);
+            for (int r = 0; r < RT; ++r) a[r] = *(const bf16x8 *)(sin + (r * 32 + i) * LDI + k0 + kh);
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+                const int nn = wave * (CT * 32) + c * 32 + i;
+                b[c] = *(const bf16x8 *)(Wl + (int64_t)nn * K + k0 + kh);
+            }
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int c = 0; c < CT; ++c)
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[r], b[c], acc[r][c], 0, 0, 0);
+        }
     }
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
         const int col = wave * (CT * 32) + c * 32 + i;
         const float bias = bl[col];
+        const float scale = IN8 ? ws[col] : 1.0f;
 #pragma unroll
         for (int r = 0; r < RT; ++r)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const int row = r * 32 + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
-                sout[row * LDO + col] = f2bf(gelu(acc[r][c][v] + bias));
+                const int row = r * 32 + 8 * (v / 4) + 4 * h + (v % 4);
+                const float g = gelu(acc[r][c][v] * scale + bias);
+                if constexpr (OUT8) ((uint8_t *)sout_)[row * LDO + col] = f2e4m3_hw(g);
+                else ((bf16_t *)sout_)[row * LDO + col] = f2bf(g);
             }
     }
 }
 
 // One block of 256 threads advances TM cells: features (T, log p, log Y) ->
 // normalised BF16 -> hidden layers -> increments (dT, dlog Y) -> new state,
-// gated on the dt match, the input box (OOD) and the element-mass change.
-template <int TM, int WD, int KIN, int NOUTP>
+// gated on the dt match, the input box (OOD), the element-mass change and,
+// with a peer network, the two networks' agreement (the state then takes the
+// mean of both increments).  FP8: the hidden-to-hidden layers run in FP8.
+template <int TM, int WD, int KIN, int NOUTP, bool FP8>
 __global__ __launch_bounds__(256) void mlp_infer(const InferArgs a) {
-    constexpr int LD = WD + 8;  // LDS row stride (bf16) of the activation buffers
+    constexpr int LD = WD + 8;   // LDS row stride (bf16) of the activation buffers
+    constexpr int LD8 = WD + 16; // the same buffers holding FP8 rows (bytes)
+    static_assert(TM * LD8 <= TM * LD * 2, "FP8 rows must fit the buffer");
     __shared__ __attribute__((aligned(16))) bf16_t buf[2][TM * LD];
     __shared__ int sfail[TM];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -361,11 +446,23 @@ __global__ __launch_bounds__(256) void mlp_infer(const InferArgs a) {
     }
     __syncthreads();
     // ---- hidden layers (ping-pong between the two buffers) ----
-    mlp_layer<TM, WD, KIN, LD, LD>(buf[0], buf[1], a.W + a.woff[0], a.b + a.boff[0]);
+    const bool l1_fp8 = FP8 && a.depth > 1;
+    if (l1_fp8) mlp_layer<TM, WD, KIN, LD, LD8, false, true>(buf[0], buf[1], a.W + a.woff[0], nullptr, a.b + a.boff[0]);
+    else mlp_layer<TM, WD, KIN, LD, LD, false, false>(buf[0], buf[1], a.W + a.woff[0], nullptr, a.b + a.boff[0]);
     __syncthreads();
     int cur = 1;
     for (int l = 1; l < a.depth; ++l) {
-        mlp_layer<TM, WD, WD, LD, LD>(buf[cur], buf[cur ^ 1], a.W + a.woff[l], a.b + a.boff[l]);
+        if constexpr (FP8) {
+            if (l + 1 < a.depth)
+                mlp_layer<TM, WD, WD, LD8, LD8, true, true>(buf[cur], buf[cur ^ 1], a.W8 + a.w8off[l],
+                                                             a.s8 + a.s8off[l], a.b + a.boff[l]);
+            else
+                mlp_layer<TM, WD, WD, LD8, LD, true, false>(buf[cur], buf[cur ^ 1], a.W8 + a.w8off[l],
+                                                             a.s8 + a.s8off[l], a.b + a.boff[l]);
+        } else {
+            mlp_layer<TM, WD, WD, LD, LD, false, false>(buf[cur], buf[cur ^ 1], a.W + a.woff[l],
+                                                         nullptr, a.b + a.boff[l]);
+        }
         __syncthreads();
         cur ^= 1;
     }
@@ -401,13 +498,19 @@ __global__ __launch_bounds__(256) void mlp_infer(const InferArgs a) {
     if (cell >= a.n) return;
     const float *o = sout + tid * LDF;
     if (a.mode == 1) {
-        for (int k = 0; k < nout; ++k) a.out32[cell * nout + k] = o[k] * a.out_std[k] + a.out_mean[k];
+        for (int k = 0; k < nout; ++k) a.out32[cell * nout + k] = out_increment(a, o, k);
         return;
     }
     bool ok = sfail[tid] == 0;
     ok = ok && fabs(a.dt[cell] * a.dtFactor - a.dt_train) <= 1e-9 * a.dt_train;
     const double T0 = a.T[cell];
-    const double T1 = T0 + (double)(o[0] * a.out_std[0] + a.out_mean[0]);
+    const float *q = a.peer ? a.peer + cell * nout : nullptr;
+    double dT = out_increment(a, o, 0);
+    if (q) {
+        ok = ok && fabs(dT - (double)q[0]) <= a.peer_tol * T0;
+        dT = 0.5 * (dT + (double)q[0]);
+    }
+    const double T1 = T0 + dT;
     ok = ok && T1 >= a.T_low && T1 <= a.T_high;
     // mass fractions from the log increments (kept in the free activation
     // buffer as [TM][64] doubles), then renormalised
@@ -415,8 +518,15 @@ __global__ __launch_bounds__(256) void mlp_infer(const InferArgs a) {
     static_assert(TM * 64 * 8 <= TM * LD * 2, "mass-fraction tile must fit the free buffer");
     double ys = 0.0;
     for (int k = 0; k < a.ns; ++k) {
-        const double y0 = fmax(a.Y[(int64_t)k * a.ldY + cell], kYFloor);
-        const double v = y0 * exp((double)(o[k + 1] * a.out_std[k + 1] + a.out_mean[k + 1]));
+        const double yk = a.Y[(int64_t)k * a.ldY + cell];
+        const double y0 = fmax(yk, kYFloor);
+        double d = out_increment(a, o, k + 1);
+        if (q) {
+            // the networks must agree on every species that is major now
+            if (yk >= 1e-3) ok = ok && fabs(d - (double)q[k + 1]) <= a.peer_tol;
+            d = 0.5 * (d + (double)q[k + 1]);
+        }
+        const double v = y0 * exp(d);
         y1[k] = v;
         ys += v;
     }

@@ -20,9 +20,15 @@ using ohcm::bf16_t;
 
 namespace {
 
+// target k: increment d (dT, dln Y) or, with delta, z = sign(d) ln(1 + |d| / delta_k)
+__device__ __forceinline__ float target(double d, const float *delta, int k) {
+    if (!delta) return (float)d;
+    return (float)copysign(log1p(fabs(d) / (double)delta[k]), d);
+}
+
 __global__ void features_kernel(int64_t n, int ns, const double *T0, const double *p,
                                 const double *Y0, int64_t ldY, const double *T1,
-                                const double *Y1, float *X, float *Yt) {
+                                const double *Y1, const float *delta, float *X, float *Yt) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const int nin = ns + 2, nout = ns + 1;
@@ -31,9 +37,11 @@ __global__ void features_kernel(int64_t n, int ns, const double *T0, const doubl
     for (int k = 0; k < ns; ++k) {
         const double l0 = log(fmax(Y0[(int64_t)k * ldY + c], ohcm::kYFloor));
         X[c * nin + 2 + k] = (float)l0;
-        if (Yt) Yt[c * nout + 1 + k] = (float)(log(fmax(Y1[(int64_t)k * ldY + c], ohcm::kYFloor)) - l0);
+        if (Yt)
+            Yt[c * nout + 1 + k] =
+                target(log(fmax(Y1[(int64_t)k * ldY + c], ohcm::kYFloor)) - l0, delta, 1 + k);
     }
-    if (Yt) Yt[c * nout] = (float)(T1[c] - T0[c]);
+    if (Yt) Yt[c * nout] = target(T1[c] - T0[c], delta, 0);
 }
 
 // packed inference copy: layer rows x K (zero padded), BF16
@@ -46,6 +54,24 @@ __global__ void pack_layer(int rows, int cols, int prow, int pcol, const float *
 __global__ void pack_bias(int rows, int prow, const float *b, float *P) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e < prow) P[e] = e < rows ? b[e] : 0.0f;
+}
+
+// FP8 copy of a [rows][cols] layer: one block per row, scale = max|w| / 448
+__global__ __launch_bounds__(256) void pack_layer_fp8(int cols, const float *W, uint8_t *P,
+                                                      float *scale) {
+    __shared__ float red[256];
+    const int r = blockIdx.x, t = threadIdx.x;
+    float m = 0.0f;
+    for (int k = t; k < cols; k += 256) m = fmaxf(m, fabsf(W[(int64_t)r * cols + k]));
+    red[t] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] = fmaxf(red[t], red[t + o]);
+        __syncthreads();
+    }
+    const float sc = red[0] > 0.0f ? red[0] / 448.0f : 1.0f;
+    for (int k = t; k < cols; k += 256) P[(int64_t)r * cols + k] = ohcm::f2e4m3(W[(int64_t)r * cols + k] / sc);
+    if (t == 0) scale[r] = sc;
 }
 
 inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
@@ -68,7 +94,20 @@ struct ohc_surrogate {
     bf16_t *d_W = nullptr;
     float *d_b = nullptr;
     int woff[8] = {}, boff[8] = {};
+    // FP8 copies of the hidden-to-hidden layers (inference precision 1)
+    uint8_t *d_W8 = nullptr;
+    float *d_s8 = nullptr;
+    int w8off[8] = {}, s8off[8] = {};
+    int precision = 0;
     bool dirty = true;
+    // symmetric-log target scales (device, nout floats; unused when !has_delta)
+    float *d_delta = nullptr;
+    bool has_delta = false;
+    // second network whose agreement the gate requires
+    ohc_surrogate *peer = nullptr;
+    double peer_tol = 0.0;
+    float *d_peer_out = nullptr;
+    int64_t peer_cap = 0;
     // normalisation, OOD box, loss weights (device, float)
     float *d_norm = nullptr;  // in_mean, in_rstd, zlo, zhi [nin]; out_mean, out_std, out_rstd, w [nout]
     double dt_train = 0.0;
@@ -227,28 +266,36 @@ int pack(ohc_surrogate *s) {
         hipLaunchKernelGGL(pack_bias, dim3(nblk(prow)), dim3(256), 0, c->stream, s->rows[l], prow,
                            s->d_p + s->b_off[l], s->d_b + s->boff[l]);
         HIPCHK(c, hipGetLastError());
+        if (l >= 1 && l < s->depth) {
+            hipLaunchKernelGGL(pack_layer_fp8, dim3(s->width), dim3(256), 0, c->stream, s->width,
+                               s->d_p + s->w_off[l], s->d_W8 + s->w8off[l], s->d_s8 + s->s8off[l]);
+            HIPCHK(c, hipGetLastError());
+        }
     }
     s->dirty = false;
     return OHC_OK;
 }
 
 template <int TM, int WD, int KIN, int NOUTP>
-void launch_infer(const ohcm::InferArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL((ohcm::mlp_infer<TM, WD, KIN, NOUTP>), dim3(nblk(a.n, TM)), dim3(256), 0, st, a);
+void launch_infer(bool fp8, const ohcm::InferArgs &a, hipStream_t st) {
+    if (fp8)
+        hipLaunchKernelGGL((ohcm::mlp_infer<TM, WD, KIN, NOUTP, true>), dim3(nblk(a.n, TM)), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((ohcm::mlp_infer<TM, WD, KIN, NOUTP, false>), dim3(nblk(a.n, TM)), dim3(256), 0, st, a);
 }
 
 template <int TM, int WD>
-hipError_t infer_wd(int kin, int noutp, const ohcm::InferArgs &a, hipStream_t st) {
+hipError_t infer_wd(int kin, int noutp, bool fp8, const ohcm::InferArgs &a, hipStream_t st) {
     if (noutp == 32) {
-        if (kin == 16) launch_infer<TM, WD, 16, 32>(a, st);
-        else if (kin == 32) launch_infer<TM, WD, 32, 32>(a, st);
-        else if (kin == 48) launch_infer<TM, WD, 48, 32>(a, st);
-        else launch_infer<TM, WD, 64, 32>(a, st);
+        if (kin == 16) launch_infer<TM, WD, 16, 32>(fp8, a, st);
+        else if (kin == 32) launch_infer<TM, WD, 32, 32>(fp8, a, st);
+        else if (kin == 48) launch_infer<TM, WD, 48, 32>(fp8, a, st);
+        else launch_infer<TM, WD, 64, 32>(fp8, a, st);
     } else {
-        if (kin == 16) launch_infer<TM, WD, 16, 64>(a, st);
-        else if (kin == 32) launch_infer<TM, WD, 32, 64>(a, st);
-        else if (kin == 48) launch_infer<TM, WD, 48, 64>(a, st);
-        else launch_infer<TM, WD, 64, 64>(a, st);
+        if (kin == 16) launch_infer<TM, WD, 16, 64>(fp8, a, st);
+        else if (kin == 32) launch_infer<TM, WD, 32, 64>(fp8, a, st);
+        else if (kin == 48) launch_infer<TM, WD, 48, 64>(fp8, a, st);
+        else launch_infer<TM, WD, 64, 64>(fp8, a, st);
     }
     return hipGetLastError();
 }
@@ -266,6 +313,13 @@ int infer(ohc_surrogate *s, ohcm::InferArgs &a) {
         a.boff[l] = s->boff[l];
     }
     a.depth = s->depth;
+    a.W8 = s->d_W8;
+    a.s8 = s->d_s8;
+    for (int l = 0; l <= s->depth; ++l) {
+        a.w8off[l] = s->w8off[l];
+        a.s8off[l] = s->s8off[l];
+    }
+    a.delta = s->has_delta ? s->d_delta : nullptr;
     a.in_mean = nm(s, 0);
     a.in_rstd = nm(s, 1);
     a.zlo = nm(s, 2);
@@ -277,8 +331,9 @@ int infer(ohc_surrogate *s, ohcm::InferArgs &a) {
     a.T_low = c->opts.T_low;
     a.T_high = c->opts.T_high;
     if (a.n == 0) return OHC_OK;
-    if (s->width == 256) HIPCHK(c, (infer_wd<64, 256>(s->kin, s->noutp, a, c->stream)));
-    else HIPCHK(c, (infer_wd<32, 512>(s->kin, s->noutp, a, c->stream)));
+    const bool fp8 = s->precision == 1;
+    if (s->width == 256) HIPCHK(c, (infer_wd<64, 256>(s->kin, s->noutp, fp8, a, c->stream)));
+    else HIPCHK(c, (infer_wd<32, 512>(s->kin, s->noutp, fp8, a, c->stream)));
     return OHC_OK;
 }
 
@@ -328,6 +383,13 @@ int ohc_surrogate_create(ohc_ctx *c, int32_t width, int32_t depth, uint64_t seed
         pw += rup(prow * pcol, 64);
         pb += rup(prow, 64);
     }
+    int64_t p8 = 0, q8 = 0;
+    for (int l = 1; l < depth; ++l) {
+        s->w8off[l] = (int)p8;
+        s->s8off[l] = (int)q8;
+        p8 += rup(width * width, 256);
+        q8 += rup(width, 64);
+    }
     s->np = (int64_t)p.size();
     ohc_surrogate *sp = s.get();
     HIPCHK(c, hipMalloc((void **)&sp->d_p, 4 * sp->np));
@@ -338,6 +400,9 @@ int ohc_surrogate_create(ohc_ctx *c, int32_t width, int32_t depth, uint64_t seed
     HIPCHK(c, hipMalloc((void **)&sp->d_W, 2 * pw));
     HIPCHK(c, hipMalloc((void **)&sp->d_b, 4 * pb));
     HIPCHK(c, hipMalloc((void **)&sp->d_norm, 4 * (4 * sp->nin + 4 * sp->nout)));
+    HIPCHK(c, hipMalloc((void **)&sp->d_W8, std::max<int64_t>(p8, 256)));
+    HIPCHK(c, hipMalloc((void **)&sp->d_s8, 4 * std::max<int64_t>(q8, 64)));
+    HIPCHK(c, hipMalloc((void **)&sp->d_delta, 4 * sp->nout));
     HIPCHK(c, hipMemcpy(sp->d_p, p.data(), 4 * sp->np, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemset(sp->d_m, 0, 4 * sp->np));
     HIPCHK(c, hipMemset(sp->d_v, 0, 4 * sp->np));
@@ -375,7 +440,8 @@ int ohc_surrogate_free(ohc_surrogate *s) {
     (void)hipStreamSynchronize(s->ctx->stream);
     for (void *q : {(void *)s->d_p, (void *)s->d_g, (void *)s->d_m, (void *)s->d_v, (void *)s->d_p16,
                     (void *)s->d_W, (void *)s->d_b, (void *)s->d_norm, (void *)s->d_elemW,
-                    s->d_ws, (void *)s->d_fail})
+                    s->d_ws, (void *)s->d_fail, (void *)s->d_W8, (void *)s->d_s8,
+                    (void *)s->d_delta, (void *)s->d_peer_out})
         if (q) (void)hipFree(q);
     delete s;
     return OHC_OK;
@@ -442,7 +508,7 @@ int ohc_surrogate_features(ohc_surrogate *s, int64_t n, const double *T0, const 
     ohc_ctx *c = s->ctx;
     if (n == 0) return OHC_OK;
     hipLaunchKernelGGL(features_kernel, dim3(nblk(n)), dim3(256), 0, c->stream, n, s->ns, T0, p, Y0,
-                       ldY, T1, Y1, X, Yt);
+                       ldY, T1, Y1, s->has_delta ? s->d_delta : nullptr, X, Yt);
     HIPCHK(c, hipGetLastError());
     return OHC_OK;
 }
@@ -517,6 +583,28 @@ int ohc_chem_solve_update_surrogate(ohc_ctx *c, ohc_surrogate *s, int64_t n, dou
     }
     hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, c->stream, s->d_nfail);
     HIPCHK(c, hipGetLastError());
+    if (s->peer) {
+        // the peer's increments for every cell, from the same (unchanged) state
+        ohc_surrogate *q = s->peer;
+        if (n > q->peer_cap) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            if (q->d_peer_out) (void)hipFree(q->d_peer_out);
+            q->d_peer_out = nullptr;
+            q->peer_cap = 0;
+            HIPCHK(c, hipMalloc((void **)&q->d_peer_out, 4 * (size_t)n * q->nout));
+            q->peer_cap = n;
+        }
+        ohcm::InferArgs b;
+        std::memset(&b, 0, sizeof(b));
+        b.n = n;
+        b.T = T;
+        b.Y = Y;
+        b.p = p;
+        b.ldY = ldY;
+        b.mode = 1;
+        b.out32 = q->d_peer_out;
+        if ((rc = infer(q, b))) return rc;
+    }
     ohcm::InferArgs a;
     std::memset(&a, 0, sizeof(a));
     a.n = n;
@@ -530,6 +618,8 @@ int ohc_chem_solve_update_surrogate(ohc_ctx *c, ohc_surrogate *s, int64_t n, dou
     a.fail_list = s->d_fail;
     a.fail_count = s->d_nfail;
     a.mode = 0;
+    a.peer = s->peer ? s->peer->d_peer_out : nullptr;
+    a.peer_tol = s->peer_tol;
     if ((rc = infer(s, a))) return rc;
     unsigned int nf = 0;
     HIPCHK(c, hipMemcpyAsync(&nf, s->d_nfail, 4, hipMemcpyDeviceToHost, c->stream));
@@ -540,6 +630,39 @@ int ohc_chem_solve_update_surrogate(ohc_ctx *c, ohc_surrogate *s, int64_t n, dou
                           nullptr, n, stats, s->d_fail, (int64_t)nf);
 }
 
+int ohc_surrogate_set_target_scale(ohc_surrogate *s, const float *delta) {
+    if (!s) return fail(nullptr, OHC_ERR_ARG, "null argument");
+    ohc_ctx *c = s->ctx;
+    if (!delta) {
+        s->has_delta = false;
+        return OHC_OK;
+    }
+    for (int k = 0; k < s->nout; ++k)
+        if (!(delta[k] > 0.0f) || !std::isfinite(delta[k]))
+            return fail(c, OHC_ERR_ARG, "target scales must be finite and > 0");
+    HIPCHK(c, hipMemcpyAsync(s->d_delta, delta, 4 * s->nout, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    s->has_delta = true;
+    return OHC_OK;
+}
+
+int ohc_surrogate_set_precision(ohc_surrogate *s, int32_t precision) {
+    if (!s) return fail(nullptr, OHC_ERR_ARG, "null argument");
+    if (precision != 0 && precision != 1) return fail(s->ctx, OHC_ERR_ARG, "precision must be 0 or 1");
+    s->precision = precision;
+    return OHC_OK;
+}
+
+int ohc_surrogate_set_peer(ohc_surrogate *s, ohc_surrogate *peer, double tol) {
+    if (!s) return fail(nullptr, OHC_ERR_ARG, "null argument");
+    if (peer && (peer == s || peer->ctx != s->ctx || peer->ns != s->ns))
+        return fail(s->ctx, OHC_ERR_ARG, "the peer must be another surrogate of the same context");
+    if (peer && !(tol >= 0.0)) return fail(s->ctx, OHC_ERR_ARG, "peer tolerance must be >= 0");
+    s->peer = peer;
+    s->peer_tol = tol;
+    return OHC_OK;
+}
+
 int ohc_surrogate_save(const ohc_surrogate *s, const char *path) {
     if (!s || !path) return fail(nullptr, OHC_ERR_ARG, "null argument");
     ohc_ctx *c = s->ctx;
@@ -548,10 +671,14 @@ int ohc_surrogate_save(const ohc_surrogate *s, const char *path) {
     HIPCHK(c, hipMemcpy(nmh.data(), s->d_norm, 4 * nmh.size(), hipMemcpyDeviceToHost));
     FILE *f = std::fopen(path, "wb");
     if (!f) return fail(c, OHC_ERR_ARG, std::string("cannot write ") + path);
-    const int32_t hdr[6] = {0x4f484353, 1, s->ns, s->width, s->depth, 0};
+    std::vector<float> dl(s->nout, 0.0f);
+    if (s->has_delta) HIPCHK(c, hipMemcpy(dl.data(), s->d_delta, 4 * dl.size(), hipMemcpyDeviceToHost));
+    // format 2: header, dt_train, normalisation, target scales (0: none), parameters
+    const int32_t hdr[6] = {0x4f484353, 2, s->ns, s->width, s->depth, s->has_delta ? 1 : 0};
     bool ok = std::fwrite(hdr, sizeof hdr, 1, f) == 1 &&
               std::fwrite(&s->dt_train, 8, 1, f) == 1 &&
               std::fwrite(nmh.data(), 4, nmh.size(), f) == nmh.size() &&
+              std::fwrite(dl.data(), 4, dl.size(), f) == dl.size() &&
               std::fwrite(p.data(), 4, p.size(), f) == p.size();
     ok = (std::fclose(f) == 0) && ok;
     return ok ? OHC_OK : fail(c, OHC_ERR_ARG, std::string("write failed: ") + path);
@@ -564,16 +691,19 @@ int ohc_surrogate_load(ohc_surrogate *s, const char *path) {
     if (!f) return fail(c, OHC_ERR_ARG, std::string("cannot read ") + path);
     int32_t hdr[6];
     double dtt = 0.0;
-    std::vector<float> p(s->np), nmh(4 * s->nin + 4 * s->nout);
-    bool ok = std::fread(hdr, sizeof hdr, 1, f) == 1 && hdr[0] == 0x4f484353 && hdr[1] == 1 &&
-              hdr[2] == s->ns && hdr[3] == s->width && hdr[4] == s->depth &&
-              std::fread(&dtt, 8, 1, f) == 1 &&
+    std::vector<float> p(s->np), nmh(4 * s->nin + 4 * s->nout), dl(s->nout, 0.0f);
+    bool ok = std::fread(hdr, sizeof hdr, 1, f) == 1 && hdr[0] == 0x4f484353 &&
+              (hdr[1] == 1 || hdr[1] == 2) && hdr[2] == s->ns && hdr[3] == s->width &&
+              hdr[4] == s->depth && std::fread(&dtt, 8, 1, f) == 1 &&
               std::fread(nmh.data(), 4, nmh.size(), f) == nmh.size() &&
+              (hdr[1] == 1 || std::fread(dl.data(), 4, dl.size(), f) == dl.size()) &&
               std::fread(p.data(), 4, p.size(), f) == p.size();
     std::fclose(f);
     if (!ok) return fail(c, OHC_ERR_ARG, std::string("not a surrogate of this shape: ") + path);
     HIPCHK(c, hipMemcpy(s->d_norm, nmh.data(), 4 * nmh.size(), hipMemcpyHostToDevice));
     s->dt_train = dtt;
+    s->has_delta = hdr[1] == 2 && hdr[5] == 1;
+    if (s->has_delta) HIPCHK(c, hipMemcpy(s->d_delta, dl.data(), 4 * dl.size(), hipMemcpyHostToDevice));
     return ohc_surrogate_set_params(s, p.data());
 }
 

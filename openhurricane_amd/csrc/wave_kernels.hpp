@@ -468,6 +468,7 @@ struct WaveLDS {
     double J[N * (N + 1)];  // Jacobian rows (stride N + 1)
     double c[64], G[64], dG[64], Cp[64], H[64], dCp[64];
     double w[64];     // net production rates omega_k
+    double gam[64];   // third-body / fall-off coefficient of each reaction lane of a chunk
     double dinv[64];  // 1 / pivot of each elimination step
     int perm[64];     // pivot lane of each elimination step
 };
@@ -495,11 +496,12 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
     __syncthreads();
     for (int base = 0; base < M.nr; base += 64) {
         const int r = base + lane;
+        double gam = 0.0;
         if (r < M.nr) {
             WRxn o;
             wreaction<WITH_J>(M, r, tf, L.c, L.G, L.dG, exact, o);
             // d(omega)/dt rows of the participants: sigma_a q, and with J the
-            // row terms sigma_a dq/dc_j, sigma_a dq/dT and sigma_a gam eff_j
+            // row terms sigma_a dq/dc_j and sigma_a dq/dT
 #pragma unroll
             for (int side = 0; side < 2; ++side) {
 #pragma unroll
@@ -517,14 +519,40 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
                             if (j < o.nb) lds_add(&row[o.bi[j]], -nu * o.dqr[j]);
                         }
                         lds_add(&row[ns], nu * o.s);
-                        if (o.gam != 0.0) {
-                            const double cg = nu * o.gam;
-                            const double *ef = M.eff + (size_t)r * 64;
-#pragma unroll 8
-                            for (int j = 0; j < ns; ++j) lds_add(&row[j], cg * ef[j]);
+                    }
+                }
+            }
+            gam = o.gam;
+        }
+        if (WITH_J) {
+            // third-body / fall-off terms sigma_a gam_r eff_rj, column j in lane
+            // j: each lane updates only its own column, so plain read-modify-
+            // write suffices, one reaction of the chunk after another
+            unsigned long long tbm = __ballot(gam != 0.0);
+            if (tbm) {
+                L.gam[lane] = gam;
+                __syncthreads();
+                const int nrp = M.nrp;
+                while (tbm) {
+                    const int sl = __ffsll((long long)tbm) - 1;
+                    tbm &= tbm - 1ull;
+                    const int rr = base + sl;
+                    const double g = L.gam[sl];
+                    const double e = lane < ns ? M.eff[(size_t)rr * 64 + lane] : 0.0;
+                    const int nf = M.ri[RI_NF * nrp + rr], nb = M.ri[RI_NB * nrp + rr];
+#pragma unroll
+                    for (int i = 0; i < kWP; ++i) {
+                        if (i < nf && lane < ns) {
+                            double *pj = &L.J[M.ri[(RI_FI + i) * nrp + rr] * (N + 1) + lane];
+                            *pj += -M.rd[(RD_FNU + i) * nrp + rr] * g * e;
+                        }
+                        if (i < nb && lane < ns) {
+                            double *pj = &L.J[M.ri[(RI_BI + i) * nrp + rr] * (N + 1) + lane];
+                            *pj += M.rd[(RD_BNU + i) * nrp + rr] * g * e;
                         }
                     }
                 }
+                __syncthreads();
             }
         }
     }

@@ -35,6 +35,8 @@ class Surrogate:
         check(lib().ohc_surrogate_num_params(self._h, C.byref(n)))
         self.num_params = n.value
         self.dt = None
+        self.delta = None
+        self._peer = None
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -62,6 +64,28 @@ class Surrogate:
         check(lib().ohc_surrogate_set_norm(self._h, *ptrs, float(dt)), self.cs._h)
         self.norm = tuple(arrs)
         self.dt = dt
+
+    def set_target_scale(self, delta):
+        """Train / predict output k as sign(d) ln(1 + |d| / delta_k) of the
+        increment d (None: plain increments).  Set before features()."""
+        if delta is None:
+            check(lib().ohc_surrogate_set_target_scale(self._h, None), self.cs._h)
+            self.delta = None
+            return
+        d = np.ascontiguousarray(delta, dtype=np.float32)
+        assert d.size == self.nout
+        check(lib().ohc_surrogate_set_target_scale(self._h, d.ctypes.data_as(C.c_void_p)), self.cs._h)
+        self.delta = d
+
+    def set_precision(self, precision):
+        """Inference precision: 0 BF16, 1 FP8 hidden-to-hidden layers."""
+        check(lib().ohc_surrogate_set_precision(self._h, int(precision)), self.cs._h)
+
+    def set_peer(self, peer, tol):
+        """Gate on agreement with a second network (None: no peer)."""
+        check(lib().ohc_surrogate_set_peer(self._h, peer._h if peer is not None else None,
+                                           float(tol)), self.cs._h)
+        self._peer = peer
 
     def save(self, path):
         check(lib().ohc_surrogate_save(self._h, str(path).encode()), self.cs._h)

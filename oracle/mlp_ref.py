@@ -87,3 +87,60 @@ def predict(params, X, norm, nin, nout, width, depth, bf=True):
     _, _, out = forward_backward(params, X, np.zeros((X.shape[0], nout)), norm, nin, nout, width,
                                  depth, bf)
     return out * out_std + out_mean
+
+
+# ---- FP8 E4M3 (OCP: bias 7, largest finite 448, no infinities) ----
+def _e4m3_table():
+    b = np.arange(256)
+    e, m = (b >> 3) & 15, b & 7
+    v = np.where(e == 0, m * 2.0 ** -9, (1.0 + m / 8.0) * 2.0 ** (e - 7.0))
+    v = np.where(b & 0x80, -v, v)
+    v[(e == 15) & (m == 7)] = np.nan
+    return v
+
+
+E4M3 = _e4m3_table()
+_POS = E4M3[:0x7F]  # codes 0x00..0x7e, increasing
+
+
+def fp8(x):
+    """Round to the nearest E4M3 value (ties to the even code), saturating at
+    +-448; returns float64."""
+    a = np.abs(np.asarray(x, dtype=np.float64))
+    a = np.minimum(a, 448.0)
+    hi = np.clip(np.searchsorted(_POS, a, side="left"), 0, _POS.size - 1)
+    lo = np.clip(hi - 1, 0, _POS.size - 1)
+    dl, dh = a - _POS[lo], _POS[hi] - a
+    pick = np.where(dl < dh, lo, np.where(dh < dl, hi, np.where(lo % 2 == 0, lo, hi)))
+    return np.copysign(_POS[pick], x)
+
+
+def predict_fp8(params, X, norm, nin, nout, width, depth):
+    """Denormalised outputs of the FP8 inference: BF16 first and last layers,
+    hidden-to-hidden layers on FP8 weights (per-row scale max|w| / 448) and FP8
+    activations, FP64 sums."""
+    in_mean, in_std, out_mean, out_std, _ = norm
+    layers = unpack(np.asarray(params, dtype=np.float64), nin, nout, width, depth)
+    # the device computes 1 / std in FP32 and multiplies
+    rstd = (np.float32(1.0) / np.asarray(in_std, dtype=np.float32)).astype(np.float64)
+    a = bf16(((X - in_mean) * rstd).astype(np.float32))
+    for l in range(depth):
+        W, b = layers[l]
+        last_hidden = l == depth - 1
+        if l == 0:
+            z = a @ bf16(W).T + b
+        else:
+            sc = (np.abs(W.astype(np.float32)).max(axis=1) / np.float32(448.0)).astype(np.float32)
+            sc = np.where(sc > 0, sc, np.float32(1.0))
+            q = fp8((W.astype(np.float32) / sc[:, None]).astype(np.float32))
+            z = (a @ q.T) * sc.astype(np.float64) + b
+        h = gelu(z)
+        a = bf16(h) if last_hidden else fp8(h)
+    W, b = layers[depth]
+    out = a @ bf16(W).T + b
+    return out * out_std + out_mean
+
+
+def inverse_target(z, delta):
+    """Increment d from the transformed target z = sign(d) ln(1 + |d| / delta)."""
+    return np.sign(z) * np.expm1(np.abs(z)) * delta

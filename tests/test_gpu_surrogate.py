@@ -149,3 +149,92 @@ def test_save_load_round_trip(env, tmp_path):
     a = s.predict(g["T"], g["p"], g["Y"])
     b = s2.predict(g["T"], g["p"], g["Y"])
     assert env["torch"].equal(a, b)
+
+
+def test_fp8_inference_matches_cpu_reference(env):
+    """FP8 hidden-to-hidden layers (f8f6f4 matrix instructions) against the CPU
+    reference with the same E4M3 roundings; also reports the FP8 - BF16 gap."""
+    from mlp_ref import predict_fp8
+
+    s, X, Yt = _surrogate(env, seed=13)
+    s.fit(X, Yt, epochs=2, batch=1024)
+    g = env["g"]
+    m = 512
+    args = (g["T"][:m].contiguous(), g["p"][:m].contiguous(), g["Y"][:, :m].contiguous())
+    out16 = s.predict(*args).cpu().numpy()
+    s.set_precision(1)
+    out8 = s.predict(*args).cpu().numpy()
+    s.set_precision(0)
+    in_mean, in_std, _, _, out_mean, out_std, _ = s.norm
+    norm = (in_mean.astype(np.float64), in_std.astype(np.float64), out_mean.astype(np.float64),
+            out_std.astype(np.float64), np.ones(s.nout))
+    ref = predict_fp8(s.get_params().astype(np.float64), X[:m].cpu().numpy().astype(np.float64),
+                      norm, s.nin, s.nout, WIDTH, DEPTH)
+    err = np.abs(out8 - ref) / out_std[None, :]
+    gap = np.abs(out8 - out16) / out_std[None, :]
+    print(f"fp8 vs reference: max {err.max():.2e} median {np.median(err):.2e}; "
+          f"fp8 vs bf16: max {gap.max():.2e} median {np.median(gap):.2e}")
+    assert err.max() < 3e-2 and np.median(err) < 2e-3
+
+
+def test_target_transform(env):
+    """Targets z = sign(d) ln(1 + |d|/delta) of the plain increments d, and
+    predictions returned as increments (the inverse applied on the device)."""
+    from mlp_ref import inverse_target, predict
+
+    from openhurricane_amd.surrogate import Surrogate
+
+    g = env["g"]
+    s0, X0, D = _surrogate(env, seed=17)
+    s = Surrogate(env["cs"], width=WIDTH, depth=DEPTH, seed=17)
+    delta = np.full(s.nout, 1e-4, dtype=np.float32)
+    delta[0] = 1e-2
+    s.set_target_scale(delta)
+    X, Z = s.features(g["T"], g["p"], g["Y"], env["T1"], env["Y1"])
+    d = D.cpu().numpy().astype(np.float64)
+    zref = np.sign(d) * np.log1p(np.abs(d) / delta.astype(np.float64))
+    np.testing.assert_allclose(Z.cpu().numpy(), zref, rtol=1e-5, atol=1e-6)
+    assert env["torch"].equal(X, X0)
+    s.fit_norm(X, Z, 1e-6)
+    m = 512
+    inc = s.predict(g["T"][:m].contiguous(), g["p"][:m].contiguous(),
+                    g["Y"][:, :m].contiguous()).cpu().numpy().astype(np.float64)
+    in_mean, in_std, _, _, out_mean, out_std, _ = s.norm
+    norm = (in_mean.astype(np.float64), in_std.astype(np.float64), out_mean.astype(np.float64),
+            out_std.astype(np.float64), np.ones(s.nout))
+    zr = predict(s.get_params().astype(np.float64), X[:m].cpu().numpy().astype(np.float64), norm,
+                 s.nin, s.nout, WIDTH, DEPTH, bf=True)
+    zdev = np.sign(inc) * np.log1p(np.abs(inc) / delta)
+    assert np.abs(zdev - zr).max() / out_std.max() < 1e-2
+    np.testing.assert_allclose(inc, inverse_target(zr, delta), rtol=5e-2,
+                               atol=1e-6 * float(delta.max()))
+
+
+def test_peer_gate(env):
+    """An identical peer changes nothing (bitwise); a disagreeing peer with a
+    zero tolerance sends every cell with a major species to the integrator."""
+    from openhurricane_amd.surrogate import Surrogate
+
+    torch, g, cs, n = env["torch"], env["g"], env["cs"], env["n"]
+    s1, X, Yt = _surrogate(env, seed=19)
+    s1.fit(X[:4096], Yt[:4096], epochs=1, batch=1024)
+    twin = Surrogate(cs, width=WIDTH, depth=DEPTH, seed=20)
+    twin.fit_norm(X, Yt, 1e-6)
+    twin.set_params(s1.get_params())
+    other = Surrogate(cs, width=WIDTH, depth=DEPTH, seed=21)
+    other.fit_norm(X, Yt, 1e-6)
+
+    def run(peer, tol):
+        s1.set_peer(peer, tol)
+        T, Y = g["T"].clone(), g["Y"].clone()
+        sub = torch.zeros(n, dtype=torch.float64, device=T.device)
+        nf, _ = s1.solve_update(T, g["p"], g["rho"], Y, env["dt"], 1.0, subDt=sub, gate_tol=1e30)
+        return nf, T, Y
+
+    nf0, T0, Y0 = run(None, 0.0)
+    nf1, T1, Y1 = run(twin, 0.0)
+    assert nf1 == nf0 and torch.equal(T0, T1) and torch.equal(Y0, Y1)
+    nf2, T2, Y2 = run(other, 0.0)
+    print("fallback without peer", nf0, "with a different peer", nf2)
+    assert nf2 >= 0.99 * n
+    s1.set_peer(None, 0.0)

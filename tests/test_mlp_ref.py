@@ -41,3 +41,18 @@ def test_backprop_matches_finite_differences():
         lm = forward_backward(p - e, X, Yt, norm, nin, nout, width, depth, bf=False)[0]
         fd[i] = (lp - lm) / 2e-6
     np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-8 * np.abs(fd).max())
+
+
+def test_fp8_rounding_follows_the_e4m3_definition():
+    """OCP E4M3: 448 is the largest finite value (saturation above), 2^-9 the
+    smallest subnormal, round to nearest with ties to the even code."""
+    from mlp_ref import E4M3, fp8
+
+    assert np.nanmax(E4M3) == 448.0 and np.isnan(E4M3[0x7F]) and np.isnan(E4M3[0xFF])
+    assert E4M3[1] == 2.0 ** -9 and E4M3[8] == 2.0 ** -6 and E4M3[0x38] == 1.0
+    x = np.array([1.0625, 1.1875, 17.0, 100.0, -7.25, 1.5 * 2 ** -9, 0.5 * 2 ** -9, 500.0, -1e6,
+                  0.3, 1e-4])
+    want = np.array([1.0, 1.25, 16.0, 96.0, -7.0, 2 ** -8, 0.0, 448.0, -448.0, 0.3125, 0.0])
+    np.testing.assert_array_equal(fp8(x), want)
+    v = E4M3[~np.isnan(E4M3)]
+    np.testing.assert_array_equal(fp8(v), v)
