@@ -132,38 +132,49 @@ int build_wmech(ohc_ctx *c) {
             for (int i = 0; i < 4; ++i) sp[(og + i) * 64 + k] = gq[i];
         }
     }
+    // Table slot q holds reaction perm[q]: reactions grouped by rate type,
+    // reversibility and fall-off form (stable, so file order within a group),
+    // so that a 64-reaction chunk of the wavefront mostly takes one code path.
+    // Only the order of floating-point sums into omega and J changes.
+    std::vector<int> perm(nr);
+    for (int r = 0; r < nr; ++r) perm[r] = r;
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
+        auto key = [&](int r) { return m.kind[r] * 100 + m.rev[r] * 10 + m.fo_type[r]; };
+        return key(a) < key(b);
+    });
     std::vector<int> ri((size_t)RI_NFIELD * nrp, 0);
     std::vector<double> rd((size_t)RD_NFIELD * nrp, 0.0), eff((size_t)std::max(nr, 1) * 64, 0.0);
-    for (int r = 0; r < nr; ++r) {
-        ri[RI_NF * nrp + r] = m.nf[r];
-        ri[RI_NB * nrp + r] = m.nb[r];
-        ri[RI_REV * nrp + r] = m.rev[r];
-        ri[RI_KIND * nrp + r] = m.kind[r];
-        ri[RI_FOT * nrp + r] = m.fo_type[r];
+    for (int q = 0; q < nr; ++q) {
+        const int r = perm[q];
+        ri[RI_NF * nrp + q] = m.nf[r];
+        ri[RI_NB * nrp + q] = m.nb[r];
+        ri[RI_REV * nrp + q] = m.rev[r];
+        ri[RI_KIND * nrp + q] = m.kind[r];
+        ri[RI_FOT * nrp + q] = m.fo_type[r];
         double s = 0.0;
         for (int i = 0; i < m.nf[r]; ++i) {
             const int k = m.f_idx[r * P + i];
-            ri[(RI_FI + i) * nrp + r] = k;
-            rd[(RD_FNU + i) * nrp + r] = m.f_nu[r * P + i];
-            rd[(RD_FO + i) * nrp + r] = m.f_ord[r * P + i];
+            ri[(RI_FI + i) * nrp + q] = k;
+            rd[(RD_FNU + i) * nrp + q] = m.f_nu[r * P + i];
+            rd[(RD_FO + i) * nrp + q] = m.f_ord[r * P + i];
             s -= m.f_nu[r * P + i];
         }
         for (int i = 0; i < m.nb[r]; ++i) {
             const int k = m.b_idx[r * P + i];
-            ri[(RI_BI + i) * nrp + r] = k;
-            rd[(RD_BNU + i) * nrp + r] = m.b_nu[r * P + i];
-            rd[(RD_BO + i) * nrp + r] = m.b_ord[r * P + i];
+            ri[(RI_BI + i) * nrp + q] = k;
+            rd[(RD_BNU + i) * nrp + q] = m.b_nu[r * P + i];
+            rd[(RD_BO + i) * nrp + q] = m.b_ord[r * P + i];
             s += m.b_nu[r * P + i];
         }
         for (int i = 0; i < 3; ++i) {
-            rd[(RD_A + i) * nrp + r] = m.arr[3 * r + i];
-            rd[(RD_A0 + i) * nrp + r] = m.arr_lo[3 * r + i];
-            rd[(RD_AR + i) * nrp + r] = m.arr_rev[3 * r + i];
+            rd[(RD_A + i) * nrp + q] = m.arr[3 * r + i];
+            rd[(RD_A0 + i) * nrp + q] = m.arr_lo[3 * r + i];
+            rd[(RD_AR + i) * nrp + q] = m.arr_rev[3 * r + i];
         }
-        for (int i = 0; i < 5; ++i) rd[(RD_FOP + i) * nrp + r] = m.fo[5 * r + i];
-        rd[RD_SUMS * nrp + r] = s;
+        for (int i = 0; i < 5; ++i) rd[(RD_FOP + i) * nrp + q] = m.fo[5 * r + i];
+        rd[RD_SUMS * nrp + q] = s;
         if (m.kind[r] != OHC_RATE_ARRHENIUS)
-            for (int k = 0; k < ns; ++k) eff[(size_t)r * 64 + k] = m.eff[(size_t)r * ns + k];
+            for (int k = 0; k < ns; ++k) eff[(size_t)q * 64 + k] = m.eff[(size_t)r * ns + k];
     }
     // one device allocation, 256-byte aligned pieces
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };

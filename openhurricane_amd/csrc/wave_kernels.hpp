@@ -30,6 +30,7 @@
 namespace ohcd {
 
 constexpr int kWP = 3;  // distinct species per reaction side in the run-time tables
+static_assert(kWP == 3, "the reaction kernels multiply three participant powers");
 
 // species fields, [field][64]
 enum {
@@ -369,12 +370,16 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
         kr = rarrh(RD[RD_AR * nrp + r], RD[RD_BR * nrp + r], RD[RD_TAR * nrp + r], tf);
         if (kind == 1) kr = Mtb * kr;
     }
-    double qf = kf, qr = kr;
+    // concentration powers c^ord of every participant, computed once and
+    // reused by q, dq/dc and dq/dT (absent participants contribute 1)
+    double pf[kWP], pb[kWP];
 #pragma unroll
     for (int i = 0; i < kWP; ++i) {
-        if (i < nf) qf *= rpow(cf[i], fo[i]);
-        if (i < nb) qr *= rpow(cb[i], bo[i]);
+        pf[i] = i < nf ? rpow(cf[i], fo[i]) : 1.0;
+        pb[i] = i < nb ? rpow(cb[i], bo[i]) : 1.0;
     }
+    const double prf = pf[0] * pf[1] * pf[2], prb = pb[0] * pb[1] * pb[2];
+    const double qf = kf * prf, qr = kr * prb;
     const double q = qf - qr;
     o.q = q;
     o.nf = nf;
@@ -387,41 +392,18 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
         o.bnu[i] = bnu[i];
     }
     if (!WITH_J) return;
-    // DqDci (reactionTypes.cpp:354-445)
+    // DqDci (reactionTypes.cpp:354-445): d(c_i^ord)/dc_i times the other
+    // participants' powers
 #pragma unroll
     for (int i = 0; i < kWP; ++i) {
-        double df = kf, dr = kr;
-#pragma unroll
-        for (int j = 0; j < kWP; ++j) {
-            if (j < nf) {
-                if (j == i) {
-                    const double ord = fo[j], c = cf[j];
-                    if (ord == 1.0) {
-                    } else if (ord < 1.0) {
-                        df *= c > kTiny ? ord * pow(c, ord - 1.0) : 0.0;
-                    } else {
-                        df *= ord * rpow(c, ord - 1.0);
-                    }
-                } else {
-                    df *= rpow(cf[j], fo[j]);
-                }
-            }
-            if (j < nb) {
-                if (j == i) {
-                    const double ord = bo[j], c = cb[j];
-                    if (ord == 1.0) {
-                    } else if (ord < 1.0) {
-                        dr *= c > kTiny ? ord * pow(c, ord - 1.0) : 0.0;
-                    } else {
-                        dr *= ord * rpow(c, ord - 1.0);
-                    }
-                } else {
-                    dr *= rpow(cb[j], bo[j]);
-                }
-            }
-        }
-        o.dqf[i] = i < nf ? df : 0.0;
-        o.dqr[i] = i < nb ? dr : 0.0;
+        auto dpow = [](double c, double ord) {
+            if (ord == 1.0) return 1.0;
+            if (ord < 1.0) return c > kTiny ? ord * pow(c, ord - 1.0) : 0.0;
+            return ord * rpow(c, ord - 1.0);
+        };
+        const int i1 = (i + 1) % kWP, i2 = (i + 2) % kWP;
+        o.dqf[i] = i < nf ? kf * dpow(cf[i], fo[i]) * (pf[i1] * pf[i2]) : 0.0;
+        o.dqr[i] = i < nb ? kr * dpow(cb[i], bo[i]) * (pb[i1] * pb[i2]) : 0.0;
     }
     // third-body / fall-off terms q dln(gamma)/dc_j = gam * eff_j
     double gam = 0.0;
@@ -442,11 +424,8 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
     else dkf = (1.0 / (1.0 + Pr)) * F * rarrh_dkdT(k0, RD[RD_B0 * nrp + r], RD[RD_TA0 * nrp + r], tf);
     if (rev == 1) dkr = (dkf - kr * dKc) * rK;
     else if (rev == 2) dkr = rarrh_dkdT(kr, RD[RD_BR * nrp + r], RD[RD_TAR * nrp + r], tf);
-#pragma unroll
-    for (int i = 0; i < kWP; ++i) {
-        if (i < nf) dkf *= rpow(cf[i], fo[i]);
-        if (i < nb) dkr *= rpow(cb[i], bo[i]);
-    }
+    dkf *= prf;
+    dkr *= prb;
     double dqdT = dkf - dkr, dgdT = 0.0;
     if (kind >= 2 && Mtb > kTiny) {
         const double k0z = k0 != 0 ? k0 : kVeryTiny, kinfz = kinf != 0 ? kinf : kVeryTiny;
@@ -625,7 +604,8 @@ __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, 
             const double l = el ? a[k] * rp : 0.0;
             if (el) a[k] = l;
             // columns j >= n hold zeros in every row and stay zero, so the
-            // update needs no per-column predicate
+            // update needs no per-column predicate.  (Publishing the pivot row
+            // through LDS instead of lane reads measured 7 % slower.)
 #pragma unroll
             for (int j = k + 1; j < N; ++j) a[j] = fma(-l, rdlane(a[j], p), a[j]);
             b = fma(-l, rdlane(b, p), b);
