@@ -67,7 +67,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cells", type=int, default=1 << 20)
+    ap.add_argument("--cells", type=int, default=None,
+                    help="cells (default: 1,048,576 for h2air, the workload's own size otherwise)")
     ap.add_argument("--cpu-sample", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--schedule", type=int, default=1,
@@ -101,7 +102,7 @@ def main():
     dev = torch.device("cuda:0")
     mech = Mechanism.h2air()
     tab = mech.tables()
-    n = args.cells
+    n = args.cells or (1 << 20)
     st = h2air_states(mech.species, tab["W"], n, seed=2024)
     opts = _lib.default_opts(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6,
                              device_pointers=1, schedule=args.schedule, jacobian=args.jacobian)
@@ -252,7 +253,7 @@ def bench_synthetic(args):
     from openhurricane_amd.sampler import synthetic_workload
 
     ns, nr, seed, default_cells = SYNTH[args.workload]
-    n = args.cells if args.cells != (1 << 20) else default_cells
+    n = args.cells or default_cells
     front = 0.2 if args.workload == "syn32" else 0.3
     mech = Mechanism.synthetic(ns, nr, seed=seed)
     tab = mech.tables()

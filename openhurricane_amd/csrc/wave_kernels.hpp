@@ -444,7 +444,10 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
 // LDS work area of one wavefront.
 template <int N>
 struct WaveLDS {
-    double J[N * (N + 1)];  // Jacobian rows (stride N + 1)
+    // row stride of the Jacobian: odd, so that a lane-per-row access pattern
+    // (lane i reading row i) hits distinct LDS banks
+    static constexpr int JS = (N + 1) | 1;
+    double J[N * JS];
     double c[64], G[64], dG[64], Cp[64], H[64], dCp[64];
     double w[64];     // net production rates omega_k
     double gam[64];   // third-body / fall-off coefficient of each reaction lane of a chunk
@@ -470,7 +473,7 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
         L.dCp[lane] = dCp;
     }
     if (WITH_J)
-        for (int e = lane; e < N * (N + 1); e += 64) L.J[e] = 0.0;
+        for (int e = lane; e < N * WaveLDS<N>::JS; e += 64) L.J[e] = 0.0;
     L.w[lane] = 0.0;
     __syncthreads();
     for (int base = 0; base < M.nr; base += 64) {
@@ -491,7 +494,7 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
                     const double nu = side == 0 ? -o.fnu[i] : o.bnu[i];
                     lds_add(&L.w[a], nu * o.q);
                     if (WITH_J) {
-                        double *row = L.J + a * (N + 1);
+                        double *row = L.J + a * WaveLDS<N>::JS;
 #pragma unroll
                         for (int j = 0; j < kWP; ++j) {
                             if (j < o.nf) lds_add(&row[o.fi[j]], nu * o.dqf[j]);
@@ -522,11 +525,11 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
 #pragma unroll
                     for (int i = 0; i < kWP; ++i) {
                         if (i < nf && lane < ns) {
-                            double *pj = &L.J[M.ri[(RI_FI + i) * nrp + rr] * (N + 1) + lane];
+                            double *pj = &L.J[M.ri[(RI_FI + i) * nrp + rr] * WaveLDS<N>::JS + lane];
                             *pj += -M.rd[(RD_FNU + i) * nrp + rr] * g * e;
                         }
                         if (i < nb && lane < ns) {
-                            double *pj = &L.J[M.ri[(RI_BI + i) * nrp + rr] * (N + 1) + lane];
+                            double *pj = &L.J[M.ri[(RI_BI + i) * nrp + rr] * WaveLDS<N>::JS + lane];
                             *pj += M.rd[(RD_BNU + i) * nrp + rr] * g * e;
                         }
                     }
@@ -556,16 +559,16 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
 #pragma unroll 8
             for (int j = 0; j < ns; ++j) {
                 const double hx = FLOW == 0 ? L.H[j] : L.H[j] - kRu * T;
-                v -= L.J[j * (N + 1) + lane] * hx;
+                v -= L.J[j * WaveLDS<N>::JS + lane] * hx;
             }
             v += Cx * dT;
             jt = v * rcC;
         }
-        const double jT = sp ? L.J[lane * (N + 1) + ns] : 0.0;  // d(omega_k)/dT
+        const double jT = sp ? L.J[lane * WaveLDS<N>::JS + ns] : 0.0;  // d(omega_k)/dT
         const double tt = wsum(sp ? (Cx * w + Hx * jT) : 0.0);
         __syncthreads();
-        if (sp) L.J[ns * (N + 1) + lane] = jt;
-        if (lane == 0) L.J[ns * (N + 1) + ns] = (dcC * dT - tt) * rcC;
+        if (sp) L.J[ns * WaveLDS<N>::JS + lane] = jt;
+        if (lane == 0) L.J[ns * WaveLDS<N>::JS + ns] = (dcC * dT - tt) * rcC;
         __syncthreads();
     }
     return sp ? w : (lane == ns ? -dT : 0.0);
@@ -582,7 +585,7 @@ __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, 
     double a[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        const double v = act && j < n ? -h * J[lane * (N + 1) + j] : 0.0;
+        const double v = act && j < n ? -h * J[lane * WaveLDS<N>::JS + j] : 0.0;
         a[j] = (act && j == lane) ? v + 1.0 : v;
     }
     int step = act ? -1 : N;  // elimination step at which this row became the pivot
@@ -628,7 +631,7 @@ __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, 
 // queue and runs chemistrySource::solve / solveUpdate on it (same control
 // flow as ie_lane_kernel, uniform across the wavefront).
 template <int N, int FLOW>
-__global__ __launch_bounds__(64) void ie_wave_kernel(const WaveParams WP) {
+__global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const WaveParams WP) {
     const LaneParams &P = WP.L;
     const WMech &M = WP.M;
     const int lane = threadIdx.x, ns = M.ns, n = ns + 1;
@@ -852,7 +855,7 @@ __global__ __launch_bounds__(64) void source_wave_kernel(const SourceParams P, c
     const double ea = sp ? L.H[lane] / W - Ri * T : 0.0;
     const double cv = wsum(sp ? (L.Cp[lane] / W - Ri) * Yk : 0.0);
     const double b = 1.0 / (rho * cv);
-    const double *row = L.J + lane * (N + 1);
+    const double *row = L.J + lane * WaveLDS<N>::JS;
     if (!P.without_last) {
         if (sp) P.D[lane * P.ldD + cell] = row[lane] + row[ns] * W * (-ea) * b;
     } else {
@@ -879,7 +882,7 @@ __global__ __launch_bounds__(64) void eval_wave_kernel(const EvalParams P, const
     else f = wave_eval<N, FLOW, false>(M, L, y, exact);
     if (lane < n) P.f[lane * P.ldf + cell] = f;
     if (P.J && lane < n)
-        for (int j = 0; j < n; ++j) P.J[(lane * n + j) * P.ldJ + cell] = L.J[lane * (N + 1) + j];
+        for (int j = 0; j < n; ++j) P.J[(lane * n + j) * P.ldJ + cell] = L.J[lane * WaveLDS<N>::JS + j];
 }
 
 }  // namespace ohcd
