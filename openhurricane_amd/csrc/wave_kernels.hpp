@@ -125,20 +125,25 @@ __device__ __forceinline__ void lds_add(double *p, double v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// The general power, kept out of line: the kernels reach it only for
+// non-integer reaction orders / exponents, and inlining its dozens of call
+// sites would multiply the code the hot loop drags through the I-cache.
+__device__ __noinline__ double pow_ool(double x, double p) { return pow(x, p); }
+
 // x^p for a run-time exponent with the integer fast paths of cpow<P>.
 __device__ __forceinline__ double rpow(double x, double p) {
     if (p == 1.0) return x;
     if (p == 2.0) return x * x;
     if (p == 0.0) return 1.0;
     if (p == 3.0) return x * x * x;
-    return pow(x, p);
+    return pow_ool(x, p);
 }
 __device__ __forceinline__ double xpow(const TFac &tf, double s) {
     if (s == 1.0) return tf.X;
     if (s == -1.0) return tf.Xinv;
     if (s == 2.0) return tf.X * tf.X;
     if (s == -2.0) return tf.Xinv * tf.Xinv;
-    return pow(tf.X, s);
+    return pow_ool(tf.X, s);
 }
 // ArrheniusRR::k: A T^b exp(-Ta/T); with b = 0 or Ta = 0 the single exp gives
 // the same value as the reference's skipped pow / exp.
@@ -398,7 +403,7 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
     for (int i = 0; i < kWP; ++i) {
         auto dpow = [](double c, double ord) {
             if (ord == 1.0) return 1.0;
-            if (ord < 1.0) return c > kTiny ? ord * pow(c, ord - 1.0) : 0.0;
+            if (ord < 1.0) return c > kTiny ? ord * pow_ool(c, ord - 1.0) : 0.0;
             return ord * rpow(c, ord - 1.0);
         };
         const int i1 = (i + 1) % kWP, i2 = (i + 2) % kWP;
