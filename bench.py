@@ -69,7 +69,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=None,
                     help="cells (default: 1,048,576 for h2air, the workload's own size otherwise)")
-    ap.add_argument("--cpu-sample", type=int, default=2048)
+    ap.add_argument("--cpu-sample", type=int, default=32768,
+                    help="cells of the CPU-baseline sample (about 10-30 s of host work)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--schedule", type=int, default=1,
                     help="1: longest-first cell order from the previous step's per-cell cost")
@@ -163,6 +164,7 @@ def main():
             "flops_per_launch": fl, "newton_iters": stats["newton_iters"],
             "steps": stats["steps"], "flops_per_newton": integrator_flops(tab)["F_newton"]}
     split = None if args.no_split else split_precision(args, mech, g, T0, Y0, dt, stream)
+    ref_gpu = None if args.no_split else reference_gpu_method(mech, g, T0, Y0, dt, stream)
     cpu = None
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
@@ -179,8 +181,49 @@ def main():
                    "cells": n, "mechanism": "h2air", "integrator": "ImplicitEuler",
                    "parallelism": "single GPU"},
         "roofline": roof, "cpu_baseline": cpu, "split_precision": split,
+        "reference_gpu_method": ref_gpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def reference_gpu_method(mech, g, T0, Y0, dt, stream):
+    """The reference's own GPU integrator (explicit Euler with step doubling,
+    its default atol 1e-11 / rtol 1e-4 / 10000 steps) on the same cells and
+    GPU, second of two flow steps; deviation of T from the implicit result."""
+    import torch
+
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource
+
+    n = T0.shape[0]
+    o = _lib.default_opts(atol=1e-11, rtol=1e-4, max_steps=10000, integrator=1,
+                          device_pointers=1, schedule=1)
+    ee = ChemistrySource(mech, device=0, opts=o)
+    ee.set_stream(stream.cuda_stream)
+    sub = torch.zeros(n, dtype=torch.float64, device=T0.device)
+    T, Y = T0.clone(), Y0.clone()
+    t = 0.0
+    for _ in range(2):
+        T.copy_(T0)
+        Y.copy_(Y0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        _, st = ee.solve_update(T, g["p"], g["rho"], Y, dt, 1.0, subDt=sub, stats=True)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t1
+    # implicit result from the same state for the comparison
+    Ti, Yi = T0.clone(), Y0.clone()
+    base = _lib.default_opts(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6,
+                             device_pointers=1)
+    ie = ChemistrySource(mech, device=0, opts=base)
+    ie.set_stream(stream.cuda_stream)
+    ie.solve_update(Ti, g["p"], g["rho"], Yi, dt, 1.0, stats=False)
+    torch.cuda.synchronize()
+    d = ((T - Ti).abs() / Ti).cpu().numpy()
+    return {"value": n / t, "unit": "cells/s", "steps": st["steps"], "rhs_evals": st["rhs_evals"],
+            "max_rel_dT_vs_implicit": float(d.max()), "median_rel_dT_vs_implicit": float(np.median(d)),
+            "note": "explicit Euler (EulerCUDA) at the reference's defaults, same GPU and cells, "
+                    "second flow step; not the benchmarked path"}
 
 
 def split_precision(args, mech, g, T0, Y0, dt, stream):
