@@ -238,3 +238,67 @@ def test_peer_gate(env):
     print("fallback without peer", nf0, "with a different peer", nf2)
     assert nf2 >= 0.99 * n
     s1.set_peer(None, 0.0)
+
+
+def test_trained_pair_gains_over_plain_integrator(env):
+    """What the trained networks gain: two networks trained on integrator
+    pairs take over part of a held-out set through the agreement gate, with
+    the error of every cell bounded (fallback cells are exact); the times of
+    the gated and the plain advance are reported."""
+    import time
+
+    from openhurricane_amd.sampler import synthetic_workload
+    from openhurricane_amd.surrogate import Surrogate
+
+    torch, cs, mech = env["torch"], env["cs"], env["mech"]
+    dev = env["g"]["T"].device
+    t = mech.tables()
+    m = 1 << 18
+    st = synthetic_workload(mech.species, t["W"], m, seed=99)
+    tr = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in st.items() if k != "kind"}
+    T1, Y1 = tr["T"].clone(), tr["Y"].clone()
+    cs.solve_update(T1, tr["p"], tr["rho"], Y1, torch.full((m,), 1e-6, dtype=torch.float64,
+                                                           device=dev), 1.0)
+    delta = np.full(mech.ns + 1, 1e-4, dtype=np.float32)
+    delta[0] = 1e-2
+    nets = []
+    for k in range(2):
+        s = Surrogate(cs, width=WIDTH, depth=DEPTH, seed=40 + k)
+        s.set_target_scale(delta)
+        X, Yt = s.features(tr["T"], tr["p"], tr["Y"], T1, Y1)
+        s.fit_norm(X, Yt, 1e-6)
+        s.fit(X, Yt, epochs=8, batch=4096, lr=1e-3, lr_end=1e-5, seed=k)
+        nets.append(s)
+    nets[0].set_peer(nets[1], 1e-4)
+    g, n = env["g"], env["n"]
+    Tr, Yr = env["T1"], env["Y1"]
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
+    def plain():
+        T, Y = g["T"].clone(), g["Y"].clone()
+        cs.solve_update(T, g["p"], g["rho"], Y, env["dt"], 1.0)
+
+    def gated():
+        T, Y = g["T"].clone(), g["Y"].clone()
+        nf, _ = nets[0].solve_update(T, g["p"], g["rho"], Y, env["dt"], 1.0, gate_tol=1e-2)
+        return nf, T, Y
+
+    timed(plain)
+    t_plain, _ = timed(plain)
+    timed(gated)
+    t_gated, (nf, T, Y) = timed(gated)
+    nets[0].set_peer(None, 0.0)
+    eT = (T - Tr).abs() / Tr
+    eY = torch.where(Yr >= 1e-3, (Y - Yr).abs() / Yr, torch.zeros_like(Yr)).amax(0)
+    e = torch.maximum(eT, eY)
+    print(f"accepted {1 - nf / n:.3f} of {n} cells; plain {t_plain * 1e3:.1f} ms, gated "
+          f"{t_gated * 1e3:.1f} ms; max error {float(e.max()):.2e}, p99 "
+          f"{float(torch.quantile(e.float(), 0.99)):.2e}")
+    assert nf < n
+    assert float(e.max()) <= 2e-2
