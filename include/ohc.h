@@ -203,6 +203,18 @@ int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_
 int ohc_ctx_free(ohc_ctx *ctx);
 /* Use a caller stream (hipStream_t passed as void*) instead of the context's. */
 int ohc_ctx_set_stream(ohc_ctx *ctx, void *stream);
+/* Autotuner (no counterpart in the reference, whose GPU path uses a static
+ * grid, src/speciesTransport/chemistrySource/include/nThreadsAndBlocks.inl:66-107):
+ * times two flow steps of solveUpdate on the given sample of cells (private
+ * copies; the arguments are not modified) for persistent-grid sizes CUs x w,
+ * w from the resident blocks per CU the kernel allows downwards until a size
+ * is clearly slower, and keeps the fastest w for this context (also into
+ * *waves_per_cu, the timed step into *best_ms; both nullable).  The sample
+ * should be large enough that no single cell's integration dominates the
+ * step.  Arrays follow the context's pointer mode. */
+int ohc_ctx_autotune(ohc_ctx *ctx, int64_t n, const double *T, const double *p,
+                     const double *rho, const double *Y, int64_t ldY, const double *dt,
+                     double dtFactor, int32_t *waves_per_cu, double *best_ms);
 int ohc_sync(ohc_ctx *ctx);
 const char *ohc_last_error(const ohc_ctx *ctx);
 /* Name of the integrator kernel the context selected ("lane:<mech>" or "generic"). */

@@ -76,6 +76,8 @@ _SIGS = [
     ("ohc_ctx_create", C.c_int, [_V, C.POINTER(Opts), C.c_int32, C.POINTER(_V)]),
     ("ohc_ctx_free", C.c_int, [_V]),
     ("ohc_ctx_set_stream", C.c_int, [_V, _V]),
+    ("ohc_ctx_autotune", C.c_int, [_V, C.c_int64, _V, _V, _V, _V, C.c_int64, _V, C.c_double,
+                                   C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     ("ohc_sync", C.c_int, [_V]),
     ("ohc_last_error", C.c_char_p, [_V]),
     ("ohc_ctx_kernel_name", C.c_char_p, [_V]),

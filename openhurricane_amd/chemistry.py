@@ -206,6 +206,18 @@ class ChemistrySource:
     def set_stream(self, stream_ptr: int):
         check(lib().ohc_ctx_set_stream(self._h, C.c_void_p(stream_ptr)), self._h)
 
+    def autotune(self, T, p, rho, Y, dt, dtFactor=1.0):
+        """Pick the persistent-grid size (wavefronts per CU) by timing two
+        flow steps on these cells; returns (waves_per_cu, timed step in ms).
+        The arrays are not modified."""
+        n = T.shape[0]
+        w = C.c_int32()
+        ms = C.c_double()
+        check(lib().ohc_ctx_autotune(self._h, n, self._p(T), self._p(p), self._p(rho), self._p(Y),
+                                     _ld(Y, n), self._p(dt), float(dtFactor), C.byref(w),
+                                     C.byref(ms)), self._h)
+        return w.value, ms.value
+
     def sync(self):
         check(lib().ohc_sync(self._h), self._h)
 

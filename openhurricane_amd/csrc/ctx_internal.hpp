@@ -30,6 +30,8 @@ struct ohc_ctx {
     ohcd::WMech wm{};           // device tables of the generic kernels
     void *d_wm = nullptr;       // one allocation holding them
     int grid = 0;  // persistent integrator grid (single-wavefront blocks)
+    int cus = 0;         // compute units of the device
+    int max_per_cu = 0;  // resident integrator blocks per CU the kernel allows
     unsigned long long *d_ctl = nullptr;  // [0] queue head, [1..8] stats
     std::vector<void *> stage;            // host-pointer staging buffers
     std::vector<size_t> stage_sz;

@@ -502,6 +502,8 @@ int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_
         int per_cu = 0;
         const void *fn = c->lane ? c->lane->integrate_fn : ohcd::wave_integrate_fn(c->mech.ns + 1);
         HIPCHK(cp, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, 0));
+        c->cus = prop.multiProcessorCount;
+        c->max_per_cu = std::max(per_cu, 1);
         if (o.waves_per_cu > 0) per_cu = std::min(per_cu, o.waves_per_cu);
         if (per_cu < 1) per_cu = 1;
         c->grid = prop.multiProcessorCount * per_cu;
@@ -533,6 +535,90 @@ int ohc_ctx_set_stream(ohc_ctx *c, void *stream) {
     if (c->own_stream) HIPCHK(c, hipStreamDestroy(c->stream));
     c->stream = (hipStream_t)stream;
     c->own_stream = false;
+    return OHC_OK;
+}
+
+int ohc_ctx_autotune(ohc_ctx *c, int64_t n, const double *T, const double *p, const double *rho,
+                     const double *Y, int64_t ldY, const double *dt, double dtFactor,
+                     int32_t *waves_per_cu, double *best_ms) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (n < 1 || ldY < n || !T || !p || !rho || !Y || !dt || !(dtFactor > 0))
+        return fail(c, OHC_ERR_ARG, "bad arguments");
+    const int ns = c->mech.ns;
+    const size_t sv = sizeof(double) * (size_t)n;
+    const size_t sY = sizeof(double) * (size_t)(ldY * (ns - 1) + n);
+    const bool dev = c->opts.device_pointers != 0;
+    // private copies of the advanced state: the caller's arrays stay untouched
+    std::vector<double> hT, hY, hsub;
+    double *dT = nullptr, *dY = nullptr, *dsub = nullptr;
+    if (dev) {
+        HIPCHK(c, hipMalloc((void **)&dT, sv));
+        HIPCHK(c, hipMalloc((void **)&dY, sY));
+        HIPCHK(c, hipMalloc((void **)&dsub, sv));
+    } else {
+        hsub.assign((size_t)n, 0.0);
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    auto restore = [&]() -> int {
+        if (dev) {
+            HIPCHK(c, hipMemcpyAsync(dT, T, sv, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(dY, Y, sY, hipMemcpyDeviceToDevice, c->stream));
+        } else {
+            hT.assign(T, T + n);
+            hY.assign(Y, Y + (size_t)(ldY * (ns - 1) + n));
+        }
+        return OHC_OK;
+    };
+    auto advance = [&]() -> int {
+        return dev ? chem_integrate(c, 1, n, nullptr, dT, p, rho, nullptr, dY, ldY, dt, dtFactor,
+                                    dsub, nullptr, n, nullptr, nullptr, 0)
+                   : chem_integrate(c, 1, n, nullptr, hT.data(), p, rho, nullptr, hY.data(), ldY,
+                                    dt, dtFactor, hsub.data(), nullptr, n, nullptr, nullptr, 0);
+    };
+    const int keep = c->grid;
+    int best = 0;
+    float best_t = 0.0f;
+    // from the most resident blocks down; stop once a size is clearly slower
+    // than the best so far (the step time is unimodal in the grid size)
+    for (int w = c->max_per_cu; w >= 1 && !rc; --w) {
+        c->grid = c->cus * w;
+        if (dev) HIPCHK(c, hipMemsetAsync(dsub, 0, sv, c->stream));
+        else std::fill(hsub.begin(), hsub.end(), 0.0);
+        // first flow step sets the sub-step memory and the cell order; the
+        // second, from the same state, is timed
+        if ((rc = restore()) || (rc = advance()) || (rc = restore())) break;
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+        if ((rc = advance())) break;
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+        if (best == 0 || ms < best_t) {
+            best = w;
+            best_t = ms;
+        } else if (ms > 1.03f * best_t) {
+            break;
+        }
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (dev) {
+        (void)hipFree(dT);
+        (void)hipFree(dY);
+        (void)hipFree(dsub);
+    }
+    if (rc || best == 0) {
+        c->grid = keep;
+        return rc ? rc : fail(c, OHC_ERR_DEVICE, "autotune found no working configuration");
+    }
+    c->grid = c->cus * best;
+    c->opts.waves_per_cu = best;
+    if (waves_per_cu) *waves_per_cu = best;
+    if (best_ms) *best_ms = best_t;
     return OHC_OK;
 }
 

@@ -76,3 +76,34 @@ def test_graph_replay_generic_kernel():
     st = synthetic_workload(mech.species, t["W"], 1024, seed=33)
     name = _run(mech, st, kernel=2, n=1024)
     assert name.startswith("wave:")
+
+
+def test_autotune_keeps_results_and_inputs(h2mech):
+    """The autotuner times every persistent-grid size on a sample, leaves the
+    sample untouched, and the advance with the chosen grid gives bit for bit
+    the result of the default grid (each cell is integrated by one lane or
+    wavefront whatever the grid)."""
+    import torch
+
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource
+    from openhurricane_amd.sampler import h2air_states
+
+    dev = torch.device("cuda:0")
+    t = h2mech.tables()
+    n = 16384
+    st = h2air_states(h2mech.species, t["W"], n, seed=35)
+    g = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in st.items()}
+    dt = torch.full((n,), 1e-6, dtype=torch.float64, device=dev)
+    opts = _lib.default_opts(atol=1e-10, rtol=1e-6, device_pointers=1)
+    cs = ChemistrySource(h2mech, opts=opts)
+    T0, Y0 = g["T"].clone(), g["Y"].clone()
+    Ta, Ya = g["T"].clone(), g["Y"].clone()
+    cs.solve_update(Ta, g["p"], g["rho"], Ya, dt, 1.0)
+    w, ms = cs.autotune(g["T"], g["p"], g["rho"], g["Y"], dt, 1.0)
+    print("autotuned waves per CU", w, "step", ms, "ms")
+    assert w >= 1 and ms > 0
+    assert torch.equal(g["T"], T0) and torch.equal(g["Y"], Y0)
+    Tb, Yb = g["T"].clone(), g["Y"].clone()
+    cs.solve_update(Tb, g["p"], g["rho"], Yb, dt, 1.0)
+    assert torch.equal(Ta, Tb) and torch.equal(Ya, Yb)
