@@ -300,8 +300,9 @@ int ohc_surrogate_set_target_scale(ohc_surrogate *s, const float *delta);
 int ohc_surrogate_set_precision(ohc_surrogate *s, int32_t precision);
 /* Gate on a second network of the same context (NULL: none): a cell passes
  * only when both networks agree on dT within tol * T and on the ln Y
- * increment of every species with Y >= 1e-3 within tol; it then takes the
- * mean of the two increments.  The peer must outlive the calls. */
+ * increment of every species with Y >= 1e-3 before or after the step within
+ * tol; it then takes the mean of the two increments.  The peer must outlive
+ * the calls. */
 int ohc_surrogate_set_peer(ohc_surrogate *s, ohc_surrogate *peer, double tol);
 /* Raw features X [n][ns+2] of states (T0, p, Y0) and, when T1/Y1 are given,
  * targets Yt [n][ns+1] of the pairs (T0, Y0) -> (T1, Y1): the increments, or

@@ -520,13 +520,13 @@ __global__ __launch_bounds__(256) void mlp_infer(const InferArgs a) {
     for (int k = 0; k < a.ns; ++k) {
         const double yk = a.Y[(int64_t)k * a.ldY + cell];
         const double y0 = fmax(yk, kYFloor);
-        double d = out_increment(a, o, k + 1);
-        if (q) {
-            // the networks must agree on every species that is major now
-            if (yk >= 1e-3) ok = ok && fabs(d - (double)q[k + 1]) <= a.peer_tol;
-            d = 0.5 * (d + (double)q[k + 1]);
-        }
+        const double ds = out_increment(a, o, k + 1);
+        const double dq = q ? (double)q[k + 1] : ds;
+        const double d = q ? 0.5 * (ds + dq) : ds;
         const double v = y0 * exp(d);
+        // the networks must agree on every species that is major before or
+        // after the step
+        if (q && (yk >= 1e-3 || v >= 1e-3)) ok = ok && fabs(ds - dq) <= a.peer_tol;
         y1[k] = v;
         ys += v;
     }

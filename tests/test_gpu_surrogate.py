@@ -44,20 +44,21 @@ def env():
     return dict(mech=mech, cs=cs, g=g, T1=T1, Y1=Y1, dt=dt, n=n, torch=torch)
 
 
-def _surrogate(env, seed=3):
+def _surrogate(env, seed=3, width=WIDTH):
     from openhurricane_amd.surrogate import Surrogate
 
-    s = Surrogate(env["cs"], width=WIDTH, depth=DEPTH, seed=seed)
+    s = Surrogate(env["cs"], width=width, depth=DEPTH, seed=seed)
     g = env["g"]
     X, Yt = s.features(g["T"], g["p"], g["Y"], env["T1"], env["Y1"])
     s.fit_norm(X, Yt, 1e-6)
     return s, X, Yt
 
 
-def test_gradient_matches_cpu_reference(env):
+@pytest.mark.parametrize("width", [256, 512])
+def test_gradient_matches_cpu_reference(env, width):
     from mlp_ref import forward_backward
 
-    s, X, Yt = _surrogate(env)
+    s, X, Yt = _surrogate(env, width=width)
     m = 256
     xb, yb = X[:m].contiguous(), Yt[:m].contiguous()
     g_dev, loss_dev = s.gradient(xb, yb)
@@ -67,12 +68,12 @@ def test_gradient_matches_cpu_reference(env):
     loss_ref, g_ref, _ = forward_backward(s.get_params().astype(np.float64),
                                           xb.cpu().numpy().astype(np.float64),
                                           yb.cpu().numpy().astype(np.float64), norm, s.nin,
-                                          s.nout, WIDTH, DEPTH, bf=True)
+                                          s.nout, width, DEPTH, bf=True)
     assert loss_dev == pytest.approx(loss_ref, rel=1e-3)
     off = 0
     for l in range(DEPTH + 1):
-        i = s.nin if l == 0 else WIDTH
-        o = WIDTH if l < DEPTH else s.nout
+        i = s.nin if l == 0 else width
+        o = width if l < DEPTH else s.nout
         for sz in (o * i, o):
             a, b = g_dev[off:off + sz].astype(np.float64), g_ref[off:off + sz]
             err = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
@@ -82,10 +83,11 @@ def test_gradient_matches_cpu_reference(env):
     assert off == s.num_params
 
 
-def test_inference_matches_cpu_reference(env):
+@pytest.mark.parametrize("width", [256, 512])
+def test_inference_matches_cpu_reference(env, width):
     from mlp_ref import predict
 
-    s, X, Yt = _surrogate(env, seed=5)
+    s, X, Yt = _surrogate(env, seed=5, width=width)
     g = env["g"]
     m = 512
     out = s.predict(g["T"][:m].contiguous(), g["p"][:m].contiguous(), g["Y"][:, :m].contiguous())
@@ -93,7 +95,7 @@ def test_inference_matches_cpu_reference(env):
     norm = (in_mean.astype(np.float64), in_std.astype(np.float64), out_mean.astype(np.float64),
             out_std.astype(np.float64), np.ones(s.nout))
     ref = predict(s.get_params().astype(np.float64), X[:m].cpu().numpy().astype(np.float64), norm,
-                  s.nin, s.nout, WIDTH, DEPTH, bf=True)
+                  s.nin, s.nout, width, DEPTH, bf=True)
     err = np.abs(out.cpu().numpy() - ref) / out_std[None, :]
     print("max normalised error", err.max())
     assert err.max() < 1e-2
@@ -151,12 +153,13 @@ def test_save_load_round_trip(env, tmp_path):
     assert env["torch"].equal(a, b)
 
 
-def test_fp8_inference_matches_cpu_reference(env):
+@pytest.mark.parametrize("width", [256, 512])
+def test_fp8_inference_matches_cpu_reference(env, width):
     """FP8 hidden-to-hidden layers (f8f6f4 matrix instructions) against the CPU
     reference with the same E4M3 roundings; also reports the FP8 - BF16 gap."""
     from mlp_ref import predict_fp8
 
-    s, X, Yt = _surrogate(env, seed=13)
+    s, X, Yt = _surrogate(env, seed=13, width=width)
     s.fit(X, Yt, epochs=2, batch=1024)
     g = env["g"]
     m = 512
@@ -169,7 +172,7 @@ def test_fp8_inference_matches_cpu_reference(env):
     norm = (in_mean.astype(np.float64), in_std.astype(np.float64), out_mean.astype(np.float64),
             out_std.astype(np.float64), np.ones(s.nout))
     ref = predict_fp8(s.get_params().astype(np.float64), X[:m].cpu().numpy().astype(np.float64),
-                      norm, s.nin, s.nout, WIDTH, DEPTH)
+                      norm, s.nin, s.nout, width, DEPTH)
     err = np.abs(out8 - ref) / out_std[None, :]
     gap = np.abs(out8 - out16) / out_std[None, :]
     print(f"fp8 vs reference: max {err.max():.2e} median {np.median(err):.2e}; "
