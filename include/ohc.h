@@ -300,7 +300,7 @@ int ohc_surrogate_set_target_scale(ohc_surrogate *s, const float *delta);
 int ohc_surrogate_set_precision(ohc_surrogate *s, int32_t precision);
 /* Gate on a second network of the same context (NULL: none): a cell passes
  * only when both networks agree on dT within tol * T and on the ln Y
- * increment of every species with Y >= 1e-3 before or after the step within
+ * increment of every species with Y >= 1e-4 before or after the step within
  * tol; it then takes the mean of the two increments.  The peer must outlive
  * the calls. */
 int ohc_surrogate_set_peer(ohc_surrogate *s, ohc_surrogate *peer, double tol);

@@ -264,6 +264,7 @@ namespace ohcm {
 // ---- fused inference + gate ------------------------------------------------
 
 constexpr double kYFloor = 1e-20;  // mass-fraction floor of the log features
+constexpr double kAgreeY = 1e-4;   // species the two-network agreement check covers
 
 typedef __attribute__((__vector_size__(8 * sizeof(int)))) int i32x8;
 
@@ -524,9 +525,10 @@ __global__ __launch_bounds__(256) void mlp_infer(const InferArgs a) {
         const double dq = q ? (double)q[k + 1] : ds;
         const double d = q ? 0.5 * (ds + dq) : ds;
         const double v = y0 * exp(d);
-        // the networks must agree on every species that is major before or
-        // after the step
-        if (q && (yk >= 1e-3 || v >= 1e-3)) ok = ok && fabs(ds - dq) <= a.peer_tol;
+        // the networks must agree on every species above kAgreeY before or
+        // after the step (a decade below the 1e-3 "major" threshold, so
+        // that a species predicted just under it is still checked)
+        if (q && (yk >= kAgreeY || v >= kAgreeY)) ok = ok && fabs(ds - dq) <= a.peer_tol;
         y1[k] = v;
         ys += v;
     }

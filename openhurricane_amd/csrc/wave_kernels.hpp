@@ -586,11 +586,15 @@ template <int N>
 __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, double b) {
     const int lane = threadIdx.x;
     const bool act = lane < n;
-    const double *J = L.J;
+    // every lane loads a row that exists (lanes past the system read row 0
+    // and discard it), so the N loads issue back to back without a branch
+    // and a wait each
+    const double *Jr = L.J + (act ? lane : 0) * WaveLDS<N>::JS;
     double a[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        const double v = act && j < n ? -h * J[lane * WaveLDS<N>::JS + j] : 0.0;
+        const double x = Jr[j];
+        const double v = act && j < n ? -h * x : 0.0;
         a[j] = (act && j == lane) ? v + 1.0 : v;
     }
     int step = act ? -1 : N;  // elimination step at which this row became the pivot
@@ -613,7 +617,8 @@ __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, 
             if (el) a[k] = l;
             // columns j >= n hold zeros in every row and stay zero, so the
             // update needs no per-column predicate.  (Publishing the pivot row
-            // through LDS instead of lane reads measured 7 % slower.)
+            // through LDS instead of lane reads measured 7 % slower: at this
+            // register budget the broadcast loads serialise on their waits.)
 #pragma unroll
             for (int j = k + 1; j < N; ++j) a[j] = fma(-l, rdlane(a[j], p), a[j]);
             b = fma(-l, rdlane(b, p), b);
