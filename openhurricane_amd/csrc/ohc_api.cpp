@@ -110,7 +110,16 @@ int build_wmech(ohc_ctx *c) {
                         "generic kernels support at most 3 distinct species per reaction side");
     const int nrp = std::max(64, (nr + 63) / 64 * 64);
     std::vector<double> sp((size_t)SP_NFIELD * 64, 0.0);
-    for (int k = 0; k < 64; ++k) sp[SP_W * 64 + k] = 1.0;
+    for (int k = 0; k < 64; ++k) {
+        sp[SP_W * 64 + k] = 1.0;
+        sp[SP_EKB * 64 + k] = 1.0;
+        sp[SP_SIG * 64 + k] = 1.0;
+    }
+    if (m.has_transport)
+        for (int k = 0; k < ns; ++k) {
+            sp[SP_EKB * 64 + k] = m.ekb[k];
+            sp[SP_SIG * 64 + k] = m.sigma[k];
+        }
     for (int k = 0; k < ns; ++k) {
         const double Ri = ohc::kRu / m.W[k];
         sp[SP_W * 64 + k] = m.W[k];
@@ -723,7 +732,7 @@ int ohc_thermo_T_from_h(ohc_ctx *c, int64_t n, const double *h, const double *p,
     if (n < 0 || ldY < n) return fail(c, OHC_ERR_ARG, "bad sizes");
     if (!h || !Y || !T) return fail(c, OHC_ERR_ARG, "null array argument");
     if (n == 0) return OHC_OK;
-    if (!c->lane) return fail(c, OHC_ERR_UNSUPPORTED, "no device kernels for this mechanism");
+    if (!c->lane && !c->wave) return fail(c, OHC_ERR_UNSUPPORTED, "no device kernels for this mechanism");
     Staging st(c);
     void *dh, *dY, *dT, *dI = nullptr;
     const size_t sv = sizeof(double) * (size_t)n;
@@ -733,7 +742,8 @@ int ohc_thermo_T_from_h(ohc_ctx *c, int64_t n, const double *h, const double *p,
     if (iflag && (rc = st.map(iflag, sizeof(int32_t) * (size_t)n, false, true, &dI))) return rc;
     ohcd::ThParams P{n, (const double *)dh, (const double *)dY, ldY, (double *)dT, (int *)dI,
                      c->opts.T_low, c->opts.T_high};
-    HIPCHK(c, c->lane->tfromh(P, c->stream));
+    if (c->lane) HIPCHK(c, c->lane->tfromh(P, c->stream));
+    else HIPCHK(c, ohcd::wave_tfromh(P, c->wm, c->stream));
     return st.finish();
 }
 
@@ -748,7 +758,9 @@ int ohc_mix_props(ohc_ctx *c, int64_t n, const double *T, const double *p, const
     if ((mu || kappa || Dk) && !c->mech.has_transport)
         return fail(c, OHC_ERR_ARG, "mechanism has no transport data");
     if (n == 0) return OHC_OK;
-    if (!c->lane || ((mu || kappa || Dk) && !c->lane_transport))
+    if (c->lane && (mu || kappa || Dk) && !c->lane_transport)
+        return fail(c, OHC_ERR_UNSUPPORTED, "no device property kernels for this mechanism");
+    if (!c->lane && !c->wave)
         return fail(c, OHC_ERR_UNSUPPORTED, "no device property kernels for this mechanism");
     Staging st(c);
     const size_t sv = sizeof(double) * (size_t)n, sD = sizeof(double) * (size_t)(ldD * (ns - 1) + n);
@@ -766,7 +778,8 @@ int ohc_mix_props(ohc_ctx *c, int64_t n, const double *T, const double *p, const
                        ldY,        (double *)dmu,      (double *)dk,       (double *)dD,
                        ldD,        (double *)dcp,      (double *)dha,      (double *)dhk,
                        c->opts.precision};
-    HIPCHK(c, c->lane->props(P, c->stream));
+    if (c->lane) HIPCHK(c, c->lane->props(P, c->stream));
+    else HIPCHK(c, ohcd::wave_props(P, c->wm, c->stream));
     return st.finish();
 }
 

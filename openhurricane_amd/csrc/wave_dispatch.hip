@@ -63,6 +63,154 @@ hipError_t wave_source(const SourceParams &p, const WMech &m, hipStream_t s) {
     return e ? e->source(p, m, s) : hipErrorInvalidValue;
 }
 
+// ---- mixture thermo + transport for run-time tables ----------------------
+// One wavefront per cell, species k in lane k; the mixture rules are wavefront
+// reductions (wsum) and the pair sums of Wilke's rule and of the
+// mixture-averaged diffusivities run in lane i over j with the other
+// species' values broadcast from LDS.  Same formulas as props_lane_kernel
+// (JANAF.cpp:155-189, speciesList.cpp:208-222, kineticTheory.cpp:37-168,
+// kineticTheory.inl:32-34, transportList.cpp:309-333), FP64.
+constexpr int kPropWaves = 4;  // cells (wavefronts) per 256-thread block
+
+__global__ __launch_bounds__(256) void props_wave_kernel(const PropParams P, const WMech M) {
+    __shared__ double s_mu[kPropWaves][64], s_x[kPropWaves][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t cell = (int64_t)blockIdx.x * kPropWaves + wv;
+    if (cell >= P.n) return;  // uniform per wavefront; no block barrier below
+    const int ns = M.ns;
+    const bool sp = lane < ns;
+    const double *S = M.sp;
+    const double T = P.T[cell], p = P.p[cell];
+    const TFac tf = tfac(T);
+    const double W = S[SP_W * 64 + lane];
+    const double y = sp ? P.Y[lane * P.ldY + cell] : 0.0;
+    double cpk = 0.0, hk = 0.0;
+    if (sp) {
+        double G, dG, Cp, H, dCp;
+        wthermo(M, lane, tf, G, dG, Cp, H, dCp);
+        cpk = Cp / W;
+        hk = H / W;
+        if (P.hk) P.hk[lane * P.ldD + cell] = hk;
+    }
+    const double cpm = wsum(cpk * y), ham = wsum(hk * y);
+    if (lane == 0) {
+        if (P.cp) P.cp[cell] = cpm;
+        if (P.ha) P.ha[cell] = ham;
+    }
+    if (!(P.mu || P.kappa || P.Dk)) return;
+    // Yi2Xi
+    double x;
+    if (ns == 1) {
+        x = sp ? 1.0 : 0.0;
+    } else {
+        const double wm = 1.0 / fmax(wsum(sp ? y * (1.0 / W) : 0.0), kVeryTiny);
+        x = sp ? fmax(0.0, fmin(1.0, y * wm / W)) : 0.0;
+        const double xt = wsum(x);
+        if (xt != 1.0) x *= 1.0 / xt;
+    }
+    // species viscosity and conductivity
+    const double ekb = S[SP_EKB * 64 + lane], sig = S[SP_SIG * 64 + lane];
+    double mu = 0.0, ka = 0.0;
+    if (sp) {
+        const double Ts = T * (1.0 / ekb);
+        const double om = 1.147 * exp(-0.145 * log(Ts)) + 1.0 / ((Ts + 0.5) * (Ts + 0.5));
+        mu = 2.6693e-6 * sqrt(W * T) / ((sig * sig) * om);
+        const double Ri = kRu / W;
+        ka = mu * Ri * (cpk / Ri + 1.25);
+    }
+    s_mu[wv][lane] = mu;
+    s_x[wv][lane] = x;
+    __builtin_amdgcn_wave_barrier();
+    if (P.mu || P.kappa) {
+        double num = 0.0, nk = 0.0;
+        if (ns == 1) {
+            num = mu;
+            nk = ka;
+        } else if (sp) {
+            double phi = 0.0;
+            for (int j = 0; j < ns; ++j) {
+                const double Wj = S[SP_W * 64 + j];
+                const double phiP = (1.0 / sqrt(8.0)) / sqrt(1.0 + W / Wj);
+                const double tt = 1.0 + sqrt(mu / s_mu[wv][j] * sqrt(Wj / W));
+                phi += phiP * s_x[wv][j] * (tt * tt);
+            }
+            const double rphi = 1.0 / phi;
+            num = mu * x * rphi;
+            nk = ka * x * rphi;
+        }
+        const double mum = wsum(num), km = wsum(nk);
+        if (lane == 0) {
+            if (P.mu) P.mu[cell] = mum;
+            if (P.kappa) P.kappa[cell] = km;
+        }
+    }
+    if (P.Dk && sp) {
+        const double Tv1p5 = T * sqrt(T), rp = 1.0 / p;
+        double sum = 0.0;
+        for (int j = 0; j < ns; ++j) {
+            if (j == lane) continue;
+            const double Wj = S[SP_W * 64 + j], sj = S[SP_SIG * 64 + j], ej = S[SP_EKB * 64 + j];
+            const double sij = 0.5 * (sig + sj);
+            const double cst = 1.858e-7 * sqrt(1.0 / W + 1.0 / Wj) / (sij * sij) * kPatm;
+            const double Ts = T * (1.0 / sqrt(ekb * ej));
+            const double Od = exp(-0.145 * log(Ts)) + 1.0 / ((Ts + 0.5) * (Ts + 0.5));
+            const double Dij = Tv1p5 * cst * rp / Od;
+            sum += s_x[wv][j] / Dij;
+        }
+        double Dim = x < 1.0 - kVeryTiny ? (1.0 - x) / sum : 0.0;
+        if (isinf(Dim)) Dim = 0.0;
+        P.Dk[lane * P.ldD + cell] = Dim;
+    }
+}
+
+// thermoList::T (thermoList.cpp:504-533) for run-time tables: one lane per
+// cell, species loop over the table (uniform addresses across the lanes).
+__global__ __launch_bounds__(256) void tfromh_table_kernel(const ThParams P, const WMech M) {
+    const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= P.n) return;
+    const double hstar = P.h[cell];
+    const double T0 = P.T[cell], Ttol = T0 * 1.0e-4;
+    double Test = T0, Tnew = T0;
+    int iter = 0, flag = 1;
+    do {
+        Test = Tnew;
+        const TFac tf = tfac(Test);
+        double hm = 0.0, cpm = 0.0;
+        for (int k = 0; k < M.ns; ++k) {
+            double G, dG, Cp, H, dCp;
+            wthermo(M, k, tf, G, dG, Cp, H, dCp);
+            const double W = M.sp[SP_W * 64 + k], yk = P.Y[k * P.ldY + cell];
+            hm += H / W * yk;
+            cpm += Cp / W * yk;
+        }
+        double Tn = Test - (hm - hstar) / cpm;
+        if (Tn < P.Tlow) {
+            flag = 0;
+            Tn = P.Tlow;
+        } else if (Tn > P.Thigh) {
+            flag = 2;
+            Tn = P.Thigh;
+        } else {
+            flag = 1;
+        }
+        Tnew = Tn;
+        if (iter++ > 100) break;
+    } while (fabs(Tnew - Test) > Ttol);
+    P.T[cell] = Tnew;
+    if (P.iflag) P.iflag[cell] = flag;
+}
+
+hipError_t wave_props(const PropParams &p, const WMech &m, hipStream_t s) {
+    const unsigned grid = (unsigned)((p.n + kPropWaves - 1) / kPropWaves);
+    hipLaunchKernelGGL(props_wave_kernel, dim3(grid), dim3(64 * kPropWaves), 0, s, p, m);
+    return hipGetLastError();
+}
+
+hipError_t wave_tfromh(const ThParams &p, const WMech &m, hipStream_t s) {
+    hipLaunchKernelGGL(tfromh_table_kernel, dim3((unsigned)((p.n + 255) / 256)), dim3(256), 0, s, p, m);
+    return hipGetLastError();
+}
+
 const void *wave_integrate_fn(int n) {
     const Entry *e = pick(n);
     return e ? e->fn() : nullptr;

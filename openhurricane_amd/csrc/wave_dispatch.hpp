@@ -16,5 +16,8 @@ hipError_t wave_integrate(const WaveParams &p, int grid, hipStream_t s);
 hipError_t wave_eval(const EvalParams &p, const WMech &m, int exact, hipStream_t s);
 hipError_t wave_source(const SourceParams &p, const WMech &m, hipStream_t s);
 const void *wave_integrate_fn(int n);
+// mixture thermo / transport and T from h for run-time tables
+hipError_t wave_props(const PropParams &p, const WMech &m, hipStream_t s);
+hipError_t wave_tfromh(const ThParams &p, const WMech &m, hipStream_t s);
 
 }  // namespace ohcd

@@ -46,7 +46,9 @@ enum {
     SP_SHI = SP_SLO + 3,
     SP_GLO = SP_SHI + 3,
     SP_GHI = SP_GLO + 4,
-    SP_NFIELD = SP_GHI + 4
+    SP_EKB = SP_GHI + 4,  // transport: eps/k [K]
+    SP_SIG,               // transport: sigma [Angstrom]
+    SP_NFIELD
 };
 // reaction int fields, [field][nrp]
 enum { RI_NF, RI_NB, RI_FI, RI_BI = RI_FI + kWP, RI_REV = RI_BI + kWP, RI_KIND, RI_FOT, RI_NFIELD };

@@ -198,3 +198,43 @@ def test_generic_54_species_matches_oracle():
     print("gpu", s, "oracle", s_o)
     _check_states(T, Y, To, Yo)
     assert abs(s["newton_iters"] - s_o["newton_iters"]) <= 0.02 * s_o["newton_iters"]
+
+
+@pytest.mark.parametrize("which", ["h2air", "syn32", "syn54"])
+def test_generic_props_and_T_from_h_match_oracle(which, h2mech, h2oracle, syn32, syn32_oracle):
+    """Mixture thermo / transport (wavefront reductions, pair sums over LDS)
+    and T from h (table-driven Newton) on the run-time tables, against the
+    oracle with the tolerances of the compiled-mechanism test."""
+    from oracle import Oracle
+
+    from openhurricane_amd.chemistry import ChemistrySource, Mechanism
+    from openhurricane_amd.sampler import h2air_states, synthetic_workload
+
+    if which == "h2air":
+        mech, orc = h2mech, h2oracle
+    elif which == "syn32":
+        mech, orc = syn32, syn32_oracle
+    else:
+        mech = Mechanism.synthetic(54, 270, seed=2)
+        orc = Oracle(mech)
+    t = mech.tables()
+    o = _opts()
+    cs = ChemistrySource(mech, opts=o)
+    assert cs.kernel_name.startswith("wave:"), cs.kernel_name
+    n = 512
+    if which == "h2air":
+        st = h2air_states(mech.species, t["W"], n, seed=19)
+    else:
+        st = synthetic_workload(mech.species, t["W"], n, seed=19)
+    out = cs.mix_props(st["T"], st["p"], st["Y"], hk=True)
+    ref = orc.mix_props(st["T"], st["p"], st["Y"], hk=True)
+    for k in ["mu", "kappa", "cp", "ha"]:
+        np.testing.assert_allclose(out[k], ref[k], rtol=1e-12, atol=0, err_msg=k)
+    np.testing.assert_allclose(out["Dk"], ref["Dk"], rtol=1e-11, atol=1e-300)
+    np.testing.assert_allclose(out["hk"], ref["hk"], rtol=1e-12, atol=1e-6)
+    T0 = st["T"] + 300.0
+    Tg, fl = cs.T_from_h(ref["ha"], st["p"], st["Y"], T0, want_flag=True)
+    To, flo = orc.T_from_h(o, ref["ha"], st["p"], st["Y"], T0)
+    np.testing.assert_allclose(Tg, To, rtol=1e-12)
+    np.testing.assert_array_equal(fl, flo)
+    np.testing.assert_allclose(Tg, st["T"], rtol=1e-4)
