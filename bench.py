@@ -378,7 +378,7 @@ def bench_synthetic(args):
         from oracle import Oracle
 
         orc = Oracle(mech)
-        m = args.cpu_sample // 8
+        m = args.cpu_sample
         sl = slice(0, m)
         Th, ph, rh = (np.ascontiguousarray(st[k][sl]) for k in ("T", "p", "rho"))
         Yh = np.ascontiguousarray(st["Y"][:, sl])
