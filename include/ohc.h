@@ -163,7 +163,10 @@ typedef struct ohc_opts {
                                1: the reference GPU path -- explicit Euler with a
                                step-doubling error estimate (EulerCUDA), every
                                cell integrated, subDt the starting step, Y from
-                               c without clipping; compiled mechanisms only       */
+                               c without clipping; compiled mechanisms only;
+                               2: ImplicitTrapezoidal (ImplicitTrapezoidal.cpp:
+                               99-335), the reference's second-order implicit
+                               alternative; run-time-table (generic) kernels   */
 } ohc_opts;
 
 /* Per-call counters (summed over cells) for roofline accounting. */

@@ -642,7 +642,12 @@ __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, 
 // Persistent integrator: each wavefront takes one cell at a time from the
 // queue and runs chemistrySource::solve / solveUpdate on it (same control
 // flow as ie_lane_kernel, uniform across the wavefront).
-template <int N, int FLOW>
+// METH 0: ImplicitEuler (ImplicitEuler.cpp:40-342); METH 2: ImplicitTrapezoidal
+// (ImplicitTrapezoidal.cpp:99-335) -- Newton on y - y0 - h/2 (f0 + f(y)) with
+// I - h/2 J, error from the second difference of f times h^3/12, and f at
+// the step start (reused from the accepted attempt's error estimate, the same
+// state, instead of a fresh evaluation).
+template <int N, int FLOW, int METH = 0>
 __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const WaveParams WP) {
     const LaneParams &P = WP.L;
     const WMech &M = WP.M;
@@ -680,6 +685,10 @@ __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const Wa
         double left = dtc;
         long long csteps = 0, citer = 0;
         bool failed = false;
+        // ImplicitTrapezoidal: f at the step start (valid when fnext_ok), f at
+        // the start of the previous step
+        double fnext = 0.0, flast = 0.0;
+        bool fnext_ok = false;
         // chemistrySource::solve: while (leftTime > 0) ODE solves of the remaining interval
         while (left > 0.0 && !failed) {
             double tn = left;
@@ -701,6 +710,17 @@ __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const Wa
                 lts1 = lts0;
                 int chk = 0;
                 double yk, error;
+                double f0 = 0.0, fk = 0.0;  // ImplicitTrapezoidal: f(y0), f(yk)
+                bool last_conv = false;
+                if constexpr (METH == 2) {
+                    if (fnext_ok) {
+                        f0 = fnext;
+                    } else {
+                        ++c_rhs;
+                        f0 = wave_eval<N, FLOW, false>(M, L, y0, WP.exact_dkp);
+                    }
+                    fnext_ok = false;
+                }
                 for (;;) {
                     lts0 = dtl;
                     ++c_att;
@@ -713,8 +733,10 @@ __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const Wa
                         const double f = wave_eval<N, FLOW, true>(M, L, yk, WP.exact_dkp);
                         ++c_newt;
                         ++citer;
-                        const double rhs = -yk + y0 + dtl * f;
-                        const double d = wave_lu_solve<N>(L, n, dtl, lane < n ? rhs : 0.0);
+                        const double h = METH == 2 ? 0.5 * dtl : dtl;
+                        const double rhs = METH == 2 ? -yk + y0 + 0.5 * dtl * (f0 + f)
+                                                     : -yk + y0 + dtl * f;
+                        const double d = wave_lu_solve<N>(L, n, h, lane < n ? rhs : 0.0);
                         if (__any(lane < n && bad(d))) {
                             conv = false;
                             maxErr = 1000.0;
@@ -747,11 +769,24 @@ __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const Wa
                         }
                     }
                     if (failed) break;
+                    last_conv = conv;
                     if (!conv) {
                         error = maxErr;
                     } else {
                         double e;
-                        if (nstep == 0) {
+                        if constexpr (METH == 2) {
+                            ++c_rhs;
+                            fk = wave_eval<N, FLOW, false>(M, L, yk, WP.exact_dkp);
+                            if (nstep == 0) {
+                                e = yk - y0 - dtl * fk;
+                            } else {
+                                const double dtn = lts0, dtn1 = lts1;
+                                const double c1 = 2.0 / (dtn * (dtn + dtn1));
+                                const double c2 = 2.0 / (dtn1 * (dtn + dtn1));
+                                const double LTEF = dtn * dtn * dtn / 12.0;
+                                e = (c1 * (fk - f0) - c2 * (f0 - flast)) * LTEF;
+                            }
+                        } else if (nstep == 0) {
                             ++c_rhs;
                             const double f = wave_eval<N, FLOW, false>(M, L, yk, WP.exact_dkp);
                             e = yk - y0 - dtl * f;
@@ -794,6 +829,13 @@ __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const Wa
                 t += dtl;
                 yl1 = y0;
                 y = yk;
+                if constexpr (METH == 2) {
+                    flast = f0;
+                    // f at the new state, unless the attempt was accepted
+                    // without an error estimate (Newton stopped at its cap)
+                    fnext = fk;
+                    fnext_ok = last_conv;
+                }
                 if (error > errg)
                     dt = fmin(fmax(P.safe * pow(error, -P.alpha), P.minsc), P.maxsc) * dtl;
                 else

@@ -792,6 +792,7 @@ typedef struct {
     int n;
     double pr; /* p0 (ConstantPressure) or rho0 (ConstantVolume) */
     double J[NMAX * NMAX], f[NMAX], yk[NMAX], yTemp[NMAX], ylast0[NMAX], ylast1[NMAX];
+    double dydtLast[NMAX]; /* ImplicitTrapezoidal: f at the start of the previous step */
     double lastTimeStep[2];
     int countStep;
     int64_t steps, attempts, newton, rhs, cell_steps;
@@ -967,6 +968,149 @@ static int ie_solve(ie_ws *s, double *tn, double *dt0, double *y) {
     return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* ImplicitTrapezoidal: ODE/src/ImplicitTrapezoidal.cpp:99-335          */
+/* ------------------------------------------------------------------ */
+/* NewtonIteration, ImplicitTrapezoidal.cpp:170-250 (autoUpdateJacobian off):
+ * y - y0 - dt/2 (f0 + f(y)) = 0 with the matrix I - dt/2 J(y) */
+static double tr_newton(ie_ws *s, const double *y0, const double *f0, double dt, double *ykp1,
+                        int *conv) {
+    const int n = s->n;
+    int count = 0;
+    double maxErr = 0.0, lastErr = 0.0;
+    int M[NMAX];
+    double err[NMAX];
+    do {
+        orc_jacobian(s->m, s->o->flow_type, s->pr, ykp1, s->f, s->J, s->o->jacobian);
+        s->newton++;
+        for (int i = 0; i < n; ++i) {
+            for (int j = 0; j < n; ++j) s->J[i * n + j] = -0.5 * dt * s->J[i * n + j];
+            s->J[i * n + i] += 1.0;
+        }
+        lu_decompose(n, s->J, M);
+        for (int i = 0; i < n; ++i) err[i] = -ykp1[i] + y0[i] + 0.5 * dt * (f0[i] + s->f[i]);
+        lu_backsub(n, s->J, err, M);
+        for (int i = 0; i < n; ++i) {
+            if (isnan(err[i]) || isinf(err[i])) {
+                *conv = 0;
+                return 1000.0;
+            }
+            ykp1[i] += err[i];
+        }
+        maxErr = maxError(s, ykp1, ykp1, err);
+        if (count > 0) {
+            double diver = maxErr / dmax(lastErr, TINY);
+            if (diver > 2 && count > 5) {
+                *conv = 0;
+                return 1000.0;
+            }
+        }
+        lastErr = maxErr;
+        count++;
+        if (isnan(maxErr)) {
+            *conv = 0;
+            return 1000.0;
+        }
+        if (count > 1000) {
+            *conv = 0;
+            return maxErr;
+        }
+    } while (maxErr > 1.0);
+    *conv = 1;
+    return maxErr;
+}
+
+/* ImplicitTrapezoidal::solve(t0, y0, dydt0, dt, y), ImplicitTrapezoidal.cpp:99-142:
+ * first step y - y0 - dt f(y); later the second difference of f, dt^3/12 */
+static double tr_step(ie_ws *s, const double *y0, const double *f0, double dt, double *y) {
+    const int n = s->n;
+    double e[NMAX], f1v[NMAX];
+    memcpy(s->yk, y0, sizeof(double) * n);
+    int conv = 1;
+    double errN = tr_newton(s, y0, f0, dt, s->yk, &conv);
+    memcpy(y, s->yk, sizeof(double) * n);
+    if (!conv) return errN;
+    orc_dydt(s->m, s->o->flow_type, s->pr, s->yk, f1v);
+    s->rhs++;
+    if (s->countStep == 0) {
+        for (int i = 0; i < n; ++i) e[i] = s->yk[i] - y0[i] - dt * f1v[i];
+        return maxIncrement(s, y0, s->yk, e);
+    }
+    const double dtn = s->lastTimeStep[0], dtn1 = s->lastTimeStep[1];
+    const double c1 = 2.0 / (dtn * (dtn + dtn1));
+    const double c2 = 2.0 / (dtn1 * (dtn + dtn1));
+    const double LTEF = dtn * dtn * dtn / 12.0;
+    for (int i = 0; i < n; ++i) {
+        e[i] = c1 * (f1v[i] - f0[i]) - c2 * (f0[i] - s->dydtLast[i]);
+        e[i] *= LTEF;
+    }
+    return maxIncrement(s, y0, s->yk, e);
+}
+
+/* ImplicitTrapezoidal::adaptiveSolve, ImplicitTrapezoidal.cpp:253-292 */
+static int tr_adaptive(ie_ws *s, double *t, double *dt0, double *y) {
+    const ohc_opts *o = s->o;
+    double dt = *dt0, error = 0.0, f0[NMAX];
+    orc_dydt(s->m, o->flow_type, s->pr, y, f0);
+    s->rhs++;
+    memcpy(s->yTemp, y, sizeof(double) * s->n);
+    s->lastTimeStep[1] = s->lastTimeStep[0];
+    int count = 0;
+    do {
+        s->lastTimeStep[0] = dt;
+        s->attempts++;
+        error = tr_step(s, y, f0, dt, s->yTemp);
+        if (!checkNewY(s, &dt, y, s->yTemp) && count < 100) {
+            error = 2;
+            count++;
+            if (dt <= VERYTINY) return OHC_ERR_NUMERIC;
+            continue;
+        }
+        if (error > 1.0) {
+            double scale = dmax(o->safe_scale * pow(error, -o->alpha_inc), o->min_scale);
+            dt *= scale;
+            if (dt <= VERYTINY) return OHC_ERR_NUMERIC;
+        }
+    } while (error > 1.0);
+    memcpy(s->dydtLast, f0, sizeof(double) * s->n);
+    *t += dt;
+    memcpy(y, s->yTemp, sizeof(double) * s->n);
+    if (error > pow(o->max_scale / o->safe_scale, -1.0 / o->alpha_inc))
+        *dt0 = dmin(dmax(o->safe_scale * pow(error, -o->alpha_inc), o->min_scale), o->max_scale) *
+               dt;
+    else
+        *dt0 = dmax(o->safe_scale * o->max_scale * dt, dt);
+    s->steps++;
+    s->cell_steps++;
+    return 0;
+}
+
+/* ImplicitTrapezoidal::solve(t0, tn, dt0, y), ImplicitTrapezoidal.cpp:294-335 */
+static int tr_solve(ie_ws *s, double *tn, double *dt0, double *y) {
+    const double t0 = 0.0;
+    double t = t0, dt = 0.2 * (*dt0), dtReached = t0;
+    int last = 0;
+    for (int n = 0; n < s->o->max_steps; ++n) {
+        s->countStep = n;
+        double dtTry0 = dt;
+        if ((t + dt - *tn) * (t + dt - t0) > 0) {
+            dt = *tn - t;
+            last = 1;
+        } else if (t + dt == *tn) {
+            last = 1;
+        }
+        int rc = tr_adaptive(s, &t, &dt, y);
+        if (rc) return rc;
+        dtReached = t;
+        if ((t - *tn) * (*tn - t0) >= 0) {
+            if (n > 0 && last) *dt0 = dmin(dtTry0, *tn);
+            return 0;
+        }
+    }
+    *tn = dtReached;
+    return 0;
+}
+
 /* Per-cell driver: chemistrySource::solve(dt, f) body (:1513-1554) and
  * solve(t, dT, p, T, rho, yi) (:1300-1325).  mode 0: R output, 1: update. */
 static int cell_integrate(ie_ws *s, double T, double p, double rho, double leftTime,
@@ -980,7 +1124,7 @@ static int cell_integrate(ie_ws *s, double T, double p, double rho, double leftT
         /* ODESolver::solve, ODESolver.inl:74-90 */
         for (int i = 0; i < ns; ++i) y[i] = c[i];
         y[ns] = Tc;
-        int rc = ie_solve(s, &dtt, subDt, y);
+        int rc = s->o->integrator == 2 ? tr_solve(s, &dtt, subDt, y) : ie_solve(s, &dtt, subDt, y);
         if (rc) return rc;
         Tc = y[ns];
         for (int i = 0; i < ns; ++i) c[i] = y[i];

@@ -238,3 +238,35 @@ def test_generic_props_and_T_from_h_match_oracle(which, h2mech, h2oracle, syn32,
     np.testing.assert_allclose(Tg, To, rtol=1e-12)
     np.testing.assert_array_equal(fl, flo)
     np.testing.assert_allclose(Tg, st["T"], rtol=1e-4)
+
+
+@pytest.mark.parametrize("which,flow", [("h2air", 0), ("syn32", 0), ("syn32", 1)])
+def test_generic_trapezoidal_matches_oracle(which, flow, h2mech, h2oracle, syn32, syn32_oracle):
+    """ImplicitTrapezoidal (integrator = 2) on the generic kernels against the
+    oracle's restatement: states, step and Newton counts."""
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource
+    from openhurricane_amd.sampler import h2air_states, synthetic_workload
+
+    if which == "h2air":
+        mech, orc = h2mech, h2oracle
+        o = _lib.default_opts(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6, integrator=2,
+                              flow_type=flow)
+        n = 512
+        st = h2air_states(mech.species, mech.tables()["W"], n, seed=23)
+    else:
+        mech, orc = syn32, syn32_oracle
+        o = _opts(integrator=2, jacobian=1, atol_inc=1e-7, rtol_inc=1e-3, flow_type=flow)
+        n = 96
+        st = synthetic_workload(mech.species, mech.tables()["W"], n, seed=29)
+    cs = ChemistrySource(mech, opts=o)
+    assert cs.kernel_name.startswith("wave:"), cs.kernel_name  # auto-selected for integrator 2
+    dt = np.full(n, 1e-6)
+    To, Yo, sub_o, s_o = orc.solve_update(o, st["T"], st["p"], st["rho"], st["Y"], dt, 1.0,
+                                          nthreads=16)
+    T, Y = st["T"].copy(), st["Y"].copy()
+    sub, s = cs.solve_update(T, st["p"], st["rho"], Y, dt, 1.0)
+    print("gpu", s, "oracle", s_o)
+    _check_states(T, Y, To, Yo)
+    assert abs(s["steps"] - s_o["steps"]) <= 0.01 * s_o["steps"]
+    assert abs(s["newton_iters"] - s_o["newton_iters"]) <= 0.02 * s_o["newton_iters"]

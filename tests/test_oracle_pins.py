@@ -233,3 +233,44 @@ def test_explicit_euler_restatement_converges(h2mech, h2oracle):
         np.testing.assert_allclose(Y.sum(0), 1.0, rtol=1e-13)
         errs.append(np.max(np.abs(Y - Yr) / np.maximum(Yr, 1e-3)))
     assert errs[0] > errs[1] > errs[2], errs
+
+
+def test_trapezoidal_restatement_converges_at_second_order(h2mech, h2oracle):
+    """ImplicitTrapezoidal (integrator = 2) approaches the tight Radau
+    solution as the step tolerance tightens, and ahead of ImplicitEuler at
+    the same tolerance (second against first order)."""
+    from scipy.integrate import solve_ivp
+
+    from openhurricane_amd import _lib
+    from openhurricane_amd.sampler import h2air_states
+
+    W = h2mech.tables()["W"]
+    n, dt = 6, 1e-6
+    st = h2air_states(h2mech.species, W, n, seed=11, T_range=(1600, 2500))
+    refT, refY = np.zeros(n), np.zeros((9, n))
+    for i in range(n):
+        y0 = np.concatenate([st["rho"][i] * st["Y"][:, i] / W, [st["T"][i]]])
+        p = st["p"][i]
+        sol = solve_ivp(lambda t, y: h2oracle.dydt(y, p), (0, dt), y0, method="Radau",
+                        rtol=1e-12, atol=1e-22,
+                        jac=lambda t, y: h2oracle.jacobian(y, p, exact_dkp=True)[1])
+        y = sol.y[:, -1]
+        refT[i] = y[-1]
+        refY[:, i] = y[:-1] * W / (y[:-1] * W).sum()
+    errs, steps = {}, {}
+    for integ in (0, 2):
+        errs[integ] = []
+        for rtol in (1e-4, 1e-6, 1e-8):
+            o = _lib.default_opts(atol=1e-16, rtol=rtol, atol_inc=1e-16, rtol_inc=rtol,
+                                  integrator=integ)
+            T, Y, _, s = h2oracle.solve_update(o, st["T"], st["p"], st["rho"], st["Y"],
+                                               np.full(n, dt), 1.0, subDt=np.full(n, 1e-13))
+            errs[integ].append(max(np.max(np.abs(T - refT) / refT),
+                                   np.max(np.abs(Y - refY) / np.maximum(refY, 1e-3))))
+            steps[(integ, rtol)] = s["steps"]
+    e = errs[2]
+    print("trapezoidal", e, "implicit Euler", errs[0], steps)
+    assert e[0] > e[1] > e[2], e
+    assert e[2] < 3e-4, e
+    # fewer steps than backward Euler for the same step tolerance
+    assert steps[(2, 1e-8)] < steps[(0, 1e-8)]
