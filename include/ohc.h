@@ -166,7 +166,7 @@ typedef struct ohc_opts {
                                c without clipping; compiled mechanisms only;
                                2: ImplicitTrapezoidal (ImplicitTrapezoidal.cpp:
                                99-335), the reference's second-order implicit
-                               alternative; run-time-table (generic) kernels   */
+                               alternative; FP64                               */
 } ohc_opts;
 
 /* Per-call counters (summed over cells) for roofline accounting. */

@@ -50,10 +50,11 @@ struct LaneParams {
     unsigned long long *stats;    // see ohc_stats: [0..5] counters, [6] dt_min bits, [7] failures
     int exact_dkp;                // 1: exact d(Kp)/dT in the Jacobian
     int prec;                     // 1: FP32 rates / Gibbs / equilibrium (split precision)
-    int integrator;               // 0: ImplicitEuler; 1: explicit Euler (reference GPU path)
+    int integrator;               // 0: ImplicitEuler; 1: explicit Euler (reference GPU path);
+                                  // 2: ImplicitTrapezoidal
 };
 
-enum : int { PH_NEED = 0, PH_NEWTON = 1, PH_ERR0 = 2, PH_DONE = 3 };
+enum : int { PH_NEED = 0, PH_NEWTON = 1, PH_ERR0 = 2, PH_DONE = 3, PH_F0 = 4 };
 
 __device__ __forceinline__ double wave_min(double v) {
     for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
@@ -71,18 +72,28 @@ __device__ __forceinline__ long long wave_max(long long v) {
     return v;
 }
 
-template <class M, int FLOW, class F = double>
+// METH 0: ImplicitEuler (ImplicitEuler.cpp:40-342).  METH 2: ImplicitTrapezoidal
+// (ImplicitTrapezoidal.cpp:99-335): Newton on y - y0 - h/2 (f0 + f(y)) with
+// I - h/2 J; every converged attempt evaluates f(y) for its error estimate
+// (first step y - y0 - h f(y), later the second difference of f times
+// h^3/12), and that f(y) becomes the next step's f0 when the attempt is
+// accepted (the same state, so f0 is not evaluated again).
+template <class M, int FLOW, class F = double, int METH = 0>
 __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
     constexpr int NS = M::NS, N = NS + 1;
+    constexpr bool TR = METH == 2;
     const int lane = threadIdx.x;
     // Cold per-lane vectors live in LDS (one slot per lane, conflict-free):
-    // the step-start state y0 and the previous step-start state y_{n-1}.
+    // the step-start state y0 and the previous step-start state y_{n-1}
+    // (ImplicitTrapezoidal: s_yl holds f(yk) of the current attempt, s_f0
+    // f(y0), s_fl f at the previous step start).
     __shared__ double s_y[N][64], s_yl[N][64], s_yk[N][64];
+    __shared__ double s_f0[TR ? N : 1][64], s_fl[TR ? N : 1][64];
     // cold scalars of the state machine and the per-lane counters
     enum { C_DTC, C_LEFT, C_SUB, C_TN, C_TRY, C_LTS0, C_LTS1, C_PR, C_T, C_DT, C_DTL, C_LERR,
            C_NCTL };
     enum { S_CELLS, S_STEPS, S_ATT, S_NEWT, S_RHS, S_MAXS, S_FAIL, S_CSTEPS, S_CITER, S_CELL,
-           S_NSTEP, S_NCOUNT, S_CHK, S_LAST, S_NST };
+           S_NSTEP, S_NCOUNT, S_CHK, S_LAST, S_F0OK, S_FKOK, S_NST };
     __shared__ double s_c[C_NCTL][64];
     __shared__ long long s_n[S_NST][64];
     __shared__ double s_dtmin[64];
@@ -114,6 +125,8 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
 #define YK(i) s_yk[i][lane]
 #define CT(i) s_c[i][lane]
 #define CNT(i) s_n[i][lane]
+#define F0(i) s_f0[i][lane]
+#define FL(i) s_fl[i][lane]
 
     // The Newton iterate also lives in LDS so that only f and J occupy
     // registers through the evaluation and the factorisation.
@@ -181,6 +194,10 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
         ncount = 0;
         lastErr = 0.0;
         phase = PH_NEWTON;
+        if constexpr (TR) {
+            CNT(S_FKOK) = 0;
+            if (!CNT(S_F0OK)) phase = PH_F0;  // f(y0) first (the iterate is y0)
+        }
     };
     // ImplicitEuler::solve(t0, tn, dt0, y) loop head for step nstep; false when
     // the step budget is exhausted (tn then becomes the time reached).
@@ -239,10 +256,20 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
             action = dtl <= kVeryTiny ? A_FAIL : A_RETRY;
         } else {
             t += dtl;
+            if constexpr (TR) {
 #pragma unroll
-            for (int i = 0; i < N; ++i) {
-                YL(i) = Y0(i);
-                Y0(i) = YK(i);
+                for (int i = 0; i < N; ++i) {
+                    FL(i) = F0(i);
+                    F0(i) = YL(i);  // f(yk) of the accepted attempt = f at the new y0
+                    Y0(i) = YK(i);
+                }
+                CNT(S_F0OK) = CNT(S_FKOK);
+            } else {
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    YL(i) = Y0(i);
+                    Y0(i) = YK(i);
+                }
             }
             if (error > PAR(Q_ERRG))
                 dt = fmin(fmax(PAR(Q_SAFE) * pow(error, -PAR(Q_ALPHA)), PAR(Q_MINSC)), PAR(Q_MAXSC)) * dtl;
@@ -295,6 +322,7 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
         CT(C_LEFT) = dtc;
         CNT(S_CSTEPS) = 0;
         CNT(S_CITER) = 0;
+        CNT(S_F0OK) = 0;
         start_ode();
     };
 
@@ -318,7 +346,7 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
                 }
             }
         }
-        if (!__any(phase == PH_NEWTON || phase == PH_ERR0)) break;
+        if (!__any(phase == PH_NEWTON || phase == PH_ERR0 || phase == PH_F0)) break;
 
         // ---- the wavefront-wide heavy step: f and J at the Newton iterate ----
         // The fences keep the compiler from hoisting control-state loads above
@@ -335,7 +363,34 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
 
         bool have = false;
         double aerr = 0.0;
-        if (phase == PH_ERR0) {
+        if constexpr (TR) {
+            if (phase == PH_F0) {
+                // f at the step start (adaptiveSolve's DyDt(t, y)), then Newton
+                CNT(S_RHS) += 1;
+                sfor<0, N>([&](auto i) { F0(i) = f[i]; });
+                CNT(S_F0OK) = 1;
+                phase = PH_NEWTON;
+            } else if (phase == PH_ERR0) {
+                // ImplicitTrapezoidal::solve error (ImplicitTrapezoidal.cpp:99-142)
+                CNT(S_RHS) += 1;
+                sfor<0, N>([&](auto i) { YL(i) = f[i]; });
+                CNT(S_FKOK) = 1;
+                if (nstep == 0) {
+                    sfor<0, N>([&](auto i) { f[i] = YK(i) - Y0(i) - dtl * f[i]; });
+                } else {
+                    const double dtn = CT(C_LTS0), dtn1 = CT(C_LTS1);
+                    const double c1 = 2.0 / (dtn * (dtn + dtn1));
+                    const double c2 = 2.0 / (dtn1 * (dtn + dtn1));
+                    const double LTEF = dtn * dtn * dtn / 12.0;
+                    sfor<0, N>([&](auto i) {
+                        const double f0 = F0(i);
+                        f[i] = (c1 * (f[i] - f0) - c2 * (f0 - FL(i))) * LTEF;
+                    });
+                }
+                have = true;
+                aerr = max_increment(f);
+            }
+        } else if (phase == PH_ERR0) {
             // first step: e = y - y0 - dt f(y) (ImplicitEuler.cpp:111-120)
             CNT(S_RHS) += 1;
             sfor<0, N>([&](auto i) { f[i] = YK(i) - Y0(i) - dtl * f[i]; });
@@ -344,11 +399,12 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
         }
         if (__any(phase == PH_NEWTON)) {
             // (I - dt J) delta = -yk + y0 + dt f  (ImplicitEuler.cpp:186-198)
-            const double h = dtl;
+            const double h = TR ? 0.5 * dtl : dtl;
             sfor<0, N>([&](auto i) {
                 sfor<0, N>([&](auto j) { J[i][j] = -h * J[i][j]; });
                 J[i][i] += 1.0;
-                f[i] = -YK(i) + Y0(i) + h * f[i];
+                if constexpr (TR) f[i] = -YK(i) + Y0(i) + h * (F0(i) + f[i]);
+                else f[i] = -YK(i) + Y0(i) + h * f[i];
             });
             lu_solve<N>(J, f);
         }
@@ -392,7 +448,7 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
                         aerr = maxErr;
                     } else if (maxErr > 1.0) {
                         // iterate again at the new yk
-                    } else if (nstep == 0) {
+                    } else if (TR || nstep == 0) {
                         phase = PH_ERR0;
                     } else {
                         // second-difference LTE (ImplicitEuler.cpp:122-139)
@@ -430,6 +486,8 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
 #undef CT
 #undef CNT
 #undef PAR
+#undef F0
+#undef FL
     if (lane == 0) {
         atomicAdd(&P.stats[0], (unsigned long long)s0);
         atomicAdd(&P.stats[1], (unsigned long long)s1);
@@ -932,6 +990,9 @@ struct LaneLaunch {
         if (p.integrator == 1) {
             if (p.flow == 0) hipLaunchKernelGGL((ee_lane_kernel<M, 0>), dim3(grid), dim3(64), 0, s, p);
             else hipLaunchKernelGGL((ee_lane_kernel<M, 1>), dim3(grid), dim3(64), 0, s, p);
+        } else if (p.integrator == 2) {
+            if (p.flow == 0) hipLaunchKernelGGL((ie_lane_kernel<M, 0, double, 2>), dim3(grid), dim3(64), 0, s, p);
+            else hipLaunchKernelGGL((ie_lane_kernel<M, 1, double, 2>), dim3(grid), dim3(64), 0, s, p);
         } else if (p.prec == 0) {
             if (p.flow == 0) hipLaunchKernelGGL((ie_lane_kernel<M, 0>), dim3(grid), dim3(64), 0, s, p);
             else hipLaunchKernelGGL((ie_lane_kernel<M, 1>), dim3(grid), dim3(64), 0, s, p);

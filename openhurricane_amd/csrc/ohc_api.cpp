@@ -484,7 +484,7 @@ int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_
     std::string names;
     for (int i = 0; i < cnt; ++i) {
         names += (i ? ", " : "") + std::string(ent[i].name);
-        if (ent[i].hash == h && (o.kernel == 0 || o.kernel == 1) && o.integrator != 2) {
+        if (ent[i].hash == h && (o.kernel == 0 || o.kernel == 1)) {
             c->lane = &ent[i];
             c->lane_transport = th != 0 && ent[i].thash == th;
         }
@@ -493,9 +493,8 @@ int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_
         return fail(nullptr, OHC_ERR_UNSUPPORTED,
                     "the FP32 split precision mode needs a specialised kernel for this mechanism");
     if (o.integrator < 0 || o.integrator > 2) return fail(nullptr, OHC_ERR_ARG, "integrator must be 0, 1 or 2");
-    if (o.integrator == 2 && (o.kernel == 1 || o.precision != 0))
-        return fail(nullptr, OHC_ERR_UNSUPPORTED,
-                    "the ImplicitTrapezoidal integrator runs on the generic FP64 kernels only");
+    if (o.integrator == 2 && o.precision != 0)
+        return fail(nullptr, OHC_ERR_UNSUPPORTED, "the ImplicitTrapezoidal integrator runs in FP64 only");
     if (o.integrator == 1 && (!c->lane || o.precision != 0))
         return fail(nullptr, OHC_ERR_UNSUPPORTED,
                     "the explicit Euler comparison integrator runs on specialised FP64 kernels only");

@@ -251,7 +251,7 @@ def test_generic_trapezoidal_matches_oracle(which, flow, h2mech, h2oracle, syn32
     if which == "h2air":
         mech, orc = h2mech, h2oracle
         o = _lib.default_opts(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6, integrator=2,
-                              flow_type=flow)
+                              flow_type=flow, kernel=2)
         n = 512
         st = h2air_states(mech.species, mech.tables()["W"], n, seed=23)
     else:
@@ -260,7 +260,7 @@ def test_generic_trapezoidal_matches_oracle(which, flow, h2mech, h2oracle, syn32
         n = 96
         st = synthetic_workload(mech.species, mech.tables()["W"], n, seed=29)
     cs = ChemistrySource(mech, opts=o)
-    assert cs.kernel_name.startswith("wave:"), cs.kernel_name  # auto-selected for integrator 2
+    assert cs.kernel_name.startswith("wave:"), cs.kernel_name
     dt = np.full(n, 1e-6)
     To, Yo, sub_o, s_o = orc.solve_update(o, st["T"], st["p"], st["rho"], st["Y"], dt, 1.0,
                                           nthreads=16)

@@ -252,3 +252,31 @@ def test_explicit_euler_comparison_mode_matches_oracle(h2mech, h2oracle, states)
     R, _, _ = cs.solve(T0, p, rho, Y0, dt, 1.0)
     Ro, _, _, _ = h2oracle.solve(o, T0, p, rho, Y0, dt, 1.0, nthreads=16)
     assert np.max(np.abs(R - Ro) / (np.abs(Ro).max(axis=0) + 1e-300)) <= 1e-5
+
+
+@pytest.mark.parametrize("flow", [0, 1])
+def test_trapezoidal_lane_kernel_matches_oracle(h2mech, h2oracle, states, flow):
+    """ImplicitTrapezoidal (integrator = 2) on the compiled lane kernel: states
+    and step counts of the oracle's restatement."""
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource
+
+    st, t = states
+    n = 1024
+    o = _lib.default_opts(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6, integrator=2,
+                          flow_type=flow)
+    cs = ChemistrySource(h2mech, opts=o)
+    assert cs.kernel_name == "lane:h2air", cs.kernel_name
+    sl = slice(0, n)
+    T0, p, rho = (np.ascontiguousarray(st[k][sl]) for k in ("T", "p", "rho"))
+    Y0 = np.ascontiguousarray(st["Y"][:, sl])
+    dt = np.full(n, 1e-6)
+    To, Yo, sub_o, s_o = h2oracle.solve_update(o, T0, p, rho, Y0, dt, 1.0, nthreads=16)
+    T, Y = T0.copy(), Y0.copy()
+    sub, s = cs.solve_update(T, p, rho, Y, dt, 1.0)
+    print("gpu", s, "oracle", s_o)
+    dT, dY = check_states(T, Y, To, Yo)
+    assert dT.max() <= 1e-6 and dY.max() <= 1e-6
+    assert abs(s["steps"] - s_o["steps"]) <= 0.01 * s_o["steps"]
+    assert abs(s["newton_iters"] - s_o["newton_iters"]) <= 0.02 * s_o["newton_iters"]
+    np.testing.assert_allclose(sub, sub_o, rtol=1e-3)
