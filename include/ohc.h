@@ -166,7 +166,9 @@ typedef struct ohc_opts {
                                c without clipping; compiled mechanisms only;
                                2: ImplicitTrapezoidal (ImplicitTrapezoidal.cpp:
                                99-335), the reference's second-order implicit
-                               alternative; FP64                               */
+                               alternative; FP64;
+                               3: BDF23 at its default order 2 (BDF23.cpp:
+                               54-509); generic kernels, FP64                  */
 } ohc_opts;
 
 /* Per-call counters (summed over cells) for roofline accounting. */

@@ -27,6 +27,7 @@ constexpr double kRu = 1.380649e-23 * 6.02214076e23 * 1e3;  // J/kmol/K
 constexpr double kPatm = 1.01325e5;
 constexpr double kTiny = 1.0e-15;
 constexpr double kVeryTiny = 1.0e-300;
+constexpr double kLarge = 1.0e15;
 constexpr double kVeryLarge = 1.0e300;
 constexpr double kRootVeryLarge = 1.0e150;
 

@@ -484,7 +484,7 @@ int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_
     std::string names;
     for (int i = 0; i < cnt; ++i) {
         names += (i ? ", " : "") + std::string(ent[i].name);
-        if (ent[i].hash == h && (o.kernel == 0 || o.kernel == 1)) {
+        if (ent[i].hash == h && (o.kernel == 0 || o.kernel == 1) && o.integrator != 3) {
             c->lane = &ent[i];
             c->lane_transport = th != 0 && ent[i].thash == th;
         }
@@ -492,9 +492,11 @@ int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_
     if (!c->lane && o.precision == 1)
         return fail(nullptr, OHC_ERR_UNSUPPORTED,
                     "the FP32 split precision mode needs a specialised kernel for this mechanism");
-    if (o.integrator < 0 || o.integrator > 2) return fail(nullptr, OHC_ERR_ARG, "integrator must be 0, 1 or 2");
-    if (o.integrator == 2 && o.precision != 0)
-        return fail(nullptr, OHC_ERR_UNSUPPORTED, "the ImplicitTrapezoidal integrator runs in FP64 only");
+    if (o.integrator < 0 || o.integrator > 3) return fail(nullptr, OHC_ERR_ARG, "integrator must be 0, 1, 2 or 3");
+    if (o.integrator >= 2 && o.precision != 0)
+        return fail(nullptr, OHC_ERR_UNSUPPORTED, "the ImplicitTrapezoidal and BDF23 integrators run in FP64 only");
+    if (o.integrator == 3 && o.kernel == 1)
+        return fail(nullptr, OHC_ERR_UNSUPPORTED, "the BDF23 integrator runs on the generic kernels");
     if (o.integrator == 1 && (!c->lane || o.precision != 0))
         return fail(nullptr, OHC_ERR_UNSUPPORTED,
                     "the explicit Euler comparison integrator runs on specialised FP64 kernels only");

@@ -581,11 +581,12 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
     return sp ? w : (lane == ns ? -dT : 0.0);
 }
 
-// Solve (I - h J) x = b with the Jacobian rows in LDS: lane i owns row i;
+// Solve (diag I - h J) x = b with the Jacobian rows in LDS: lane i owns row i;
 // partial pivoting as SquareMatrix::LUDecomposeDPivoting (largest |pivot|),
 // forward substitution folded into the elimination.  Returns x_i in lane i.
 template <int N>
-__device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, double b) {
+__device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, double b,
+                                                double diag = 1.0) {
     const int lane = threadIdx.x;
     const bool act = lane < n;
     // every lane loads a row that exists (lanes past the system read row 0
@@ -597,7 +598,7 @@ __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, 
     for (int j = 0; j < N; ++j) {
         const double x = Jr[j];
         const double v = act && j < n ? -h * x : 0.0;
-        a[j] = (act && j == lane) ? v + 1.0 : v;
+        a[j] = (act && j == lane) ? v + diag : v;
     }
     int step = act ? -1 : N;  // elimination step at which this row became the pivot
 #pragma unroll
@@ -646,7 +647,12 @@ __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, 
 // (ImplicitTrapezoidal.cpp:99-335) -- Newton on y - y0 - h/2 (f0 + f(y)) with
 // I - h/2 J, error from the second difference of f times h^3/12, and f at
 // the step start (reused from the accepted attempt's error estimate, the same
-// state, instead of a fresh evaluation).
+// state, instead of a fresh evaluation); METH 3: BDF23 at its default order 2
+// (BDF23.cpp:54-509) -- variable-step BDF2 (order 1 on a solve's first step)
+// on a0 y + a1 y_n + a2 y_n-1 = h f(y), first step min(0.1 tn, 1e-8), doubled
+// on steps 4 and 5, error y - y0 - h f(y) on the first three steps and the
+// third difference of the step-start states after, and an attempt whose error
+// grew from the previous attempt's but stays below 5 accepted as it is.
 template <int N, int FLOW, int METH = 0>
 __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const WaveParams WP) {
     const LaneParams &P = WP.L;
@@ -693,11 +699,30 @@ __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const Wa
         while (left > 0.0 && !failed) {
             double tn = left;
             // ImplicitEuler::solve(t0, tn, dt0, y)
-            double t = 0.0, dt = 0.2 * subDt;
+            double t = 0.0, dt = METH == 3 ? subDt : 0.2 * subDt;
             bool last = false, done = false;
-            double yl1 = y, lts0 = 0.0, lts1 = 0.0;
+            double yl1 = y, lts0 = 0.0, lts1 = 0.0, lts2 = 0.0;
+            double h0 = y, h1 = y, h2 = y;  // BDF23 step-start history (yklast_)
             for (int nstep = 0; nstep < P.max_steps; ++nstep) {
                 const double y0 = y;
+                int order = 0;
+                if constexpr (METH == 3) {
+                    if (nstep == 0) {
+                        dt = fmin(0.1 * tn, 1e-8);
+                        order = 1;
+                        h0 = y;
+                    } else if (nstep == 1) {
+                        order = 2;
+                        h1 = h0;
+                        h0 = y;
+                    } else {
+                        order = 2;
+                        if (nstep > 3 && nstep <= 5) dt *= 2.0;
+                        h2 = h1;
+                        h1 = h0;
+                        h0 = y;
+                    }
+                }
                 const double dtTry = dt;
                 if ((t + dt - tn) * (t + dt) > 0) {
                     dt = tn - t;
@@ -707,7 +732,9 @@ __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const Wa
                 }
                 // adaptiveSolve
                 double dtl = dt;
+                lts2 = lts1;
                 lts1 = lts0;
+                double lastError = kLarge;  // BDF23
                 int chk = 0;
                 double yk, error;
                 double f0 = 0.0, fk = 0.0;  // ImplicitTrapezoidal: f(y0), f(yk)
@@ -724,6 +751,23 @@ __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const Wa
                 for (;;) {
                     lts0 = dtl;
                     ++c_att;
+                    // BDF23::setAlpha (orders 1 and 2) and the history term
+                    double ba0 = 1.0, yh = 0.0;
+                    if constexpr (METH == 3) {
+                        if (order == 1) {
+                            yh = -1.0 * h0;
+                        } else {
+                            const double D = lts0, D1 = lts1;
+                            double a1 = -(D + D1) / (D * D1), a2 = D / (D1 * (D + D1));
+                            double a0 = -(a1 + a2);
+                            a0 *= D;
+                            a1 *= D;
+                            a2 *= D;
+                            ba0 = a0;
+                            yh = a2 * h1;
+                            yh += a1 * h0;
+                        }
+                    }
                     // ImplicitEuler::solve(t, dt, y0, y): Newton iterations
                     yk = y0;
                     int count = 0;
@@ -734,9 +778,16 @@ __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const Wa
                         ++c_newt;
                         ++citer;
                         const double h = METH == 2 ? 0.5 * dtl : dtl;
-                        const double rhs = METH == 2 ? -yk + y0 + 0.5 * dtl * (f0 + f)
-                                                     : -yk + y0 + dtl * f;
-                        const double d = wave_lu_solve<N>(L, n, h, lane < n ? rhs : 0.0);
+                        double rhs, dg = 1.0;
+                        if constexpr (METH == 2) {
+                            rhs = -yk + y0 + 0.5 * dtl * (f0 + f);
+                        } else if constexpr (METH == 3) {
+                            rhs = -(ba0 * yk + yh - dtl * f);
+                            dg = ba0;
+                        } else {
+                            rhs = -yk + y0 + dtl * f;
+                        }
+                        const double d = wave_lu_solve<N>(L, n, h, lane < n ? rhs : 0.0, dg);
                         if (__any(lane < n && bad(d))) {
                             conv = false;
                             maxErr = 1000.0;
@@ -774,7 +825,18 @@ __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const Wa
                         error = maxErr;
                     } else {
                         double e;
-                        if constexpr (METH == 2) {
+                        if constexpr (METH == 3) {
+                            if (nstep <= 2) {
+                                ++c_rhs;
+                                const double f = wave_eval<N, FLOW, false>(M, L, yk, WP.exact_dkp);
+                                e = yk - y0 - dtl * f;
+                            } else {
+                                const double D = lts0, D1 = lts1, D2 = lts2;
+                                const double c1 = 1.0 / D, c2 = (1.0 / D1) * (1.0 + D / D1),
+                                             c3 = D / (D1 * D2), ltef = (D + D1) / 6.0;
+                                e = (c1 * (yk - h0) - c2 * (h0 - h1) + c3 * (h1 - h2)) * ltef;
+                            }
+                        } else if constexpr (METH == 2) {
                             ++c_rhs;
                             fk = wave_eval<N, FLOW, false>(M, L, yk, WP.exact_dkp);
                             if (nstep == 0) {
@@ -814,12 +876,16 @@ __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const Wa
                         }
                         continue;
                     }
+                    if constexpr (METH == 3) {
+                        if (lastError < error && error < 5.0) break;
+                    }
                     if (error > 1.0) {
                         dtl *= fmax(P.safe * pow(error, -P.alpha), P.minsc);
                         if (dtl <= kVeryTiny) {
                             failed = true;
                             break;
                         }
+                        lastError = error;
                         continue;
                     }
                     break;

@@ -13,7 +13,10 @@ namespace ohcd {
 
 hipError_t OHC_CAT(wave_integrate_, OHC_WAVE_N)(const WaveParams &p, int grid, hipStream_t s) {
     constexpr int N = OHC_WAVE_N;
-    if (p.L.integrator == 2) {
+    if (p.L.integrator == 3) {
+        if (p.L.flow == 0) hipLaunchKernelGGL((ie_wave_kernel<N, 0, 3>), dim3(grid), dim3(64), 0, s, p);
+        else hipLaunchKernelGGL((ie_wave_kernel<N, 1, 3>), dim3(grid), dim3(64), 0, s, p);
+    } else if (p.L.integrator == 2) {
         if (p.L.flow == 0) hipLaunchKernelGGL((ie_wave_kernel<N, 0, 2>), dim3(grid), dim3(64), 0, s, p);
         else hipLaunchKernelGGL((ie_wave_kernel<N, 1, 2>), dim3(grid), dim3(64), 0, s, p);
     } else if (p.L.flow == 0) {

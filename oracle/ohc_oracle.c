@@ -793,6 +793,10 @@ typedef struct {
     double pr; /* p0 (ConstantPressure) or rho0 (ConstantVolume) */
     double J[NMAX * NMAX], f[NMAX], yk[NMAX], yTemp[NMAX], ylast0[NMAX], ylast1[NMAX];
     double dydtLast[NMAX]; /* ImplicitTrapezoidal: f at the start of the previous step */
+    double ykl[4][NMAX];   /* BDF23: step-start states, newest first */
+    double lts4[4];        /* BDF23: lastTimeStep_[0..3] */
+    double alpha[4];       /* BDF23: coefficients of the current order */
+    int nalpha;
     double lastTimeStep[2];
     int countStep;
     int64_t steps, attempts, newton, rhs, cell_steps;
@@ -1111,6 +1115,199 @@ static int tr_solve(ie_ws *s, double *tn, double *dt0, double *y) {
     return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* BDF23 (order 2, its default): ODE/src/BDF23.cpp:54-509               */
+/* ------------------------------------------------------------------ */
+/* BDF23::setAlpha, BDF23.cpp:54-112 (orders 1 and 2) */
+static void bdf_set_alpha(ie_ws *s, int iord) {
+    if (iord == 1) {
+        s->nalpha = 2;
+        s->alpha[0] = 1.0;
+        s->alpha[1] = -1.0;
+        return;
+    }
+    const double Dtn = s->lts4[0], Dtn1 = s->lts4[1];
+    s->nalpha = 3;
+    s->alpha[1] = -(Dtn + Dtn1) / (Dtn * Dtn1);
+    s->alpha[2] = Dtn / (Dtn1 * (Dtn + Dtn1));
+    s->alpha[0] = -(s->alpha[1] + s->alpha[2]);
+    for (int k = 0; k < 3; ++k) s->alpha[k] *= Dtn;
+}
+
+/* BDF23::NewtonIteration, BDF23.cpp:283-389 (autoUpdateJacobian off):
+ * alpha0 y + yhist - dt f(y) = 0 with the matrix alpha0 I - dt J */
+static double bdf_newton(ie_ws *s, const double *yh, double dt, double *ykp1, int *conv) {
+    const int n = s->n;
+    int count = 0;
+    double maxErr = 0.0, lastErr = 0.0;
+    int M[NMAX];
+    double err[NMAX];
+    const double a0 = s->alpha[0];
+    do {
+        orc_jacobian(s->m, s->o->flow_type, s->pr, ykp1, s->f, s->J, s->o->jacobian);
+        s->newton++;
+        for (int i = 0; i < n; ++i) {
+            for (int j = 0; j < n; ++j) s->J[i * n + j] = -dt * s->J[i * n + j];
+            s->J[i * n + i] += a0;
+        }
+        lu_decompose(n, s->J, M);
+        for (int i = 0; i < n; ++i) err[i] = -(a0 * ykp1[i] + yh[i] - dt * s->f[i]);
+        lu_backsub(n, s->J, err, M);
+        for (int i = 0; i < n; ++i) {
+            if (isnan(err[i]) || isinf(err[i])) {
+                *conv = 0;
+                return 1000.0;
+            }
+            ykp1[i] += err[i];
+        }
+        maxErr = maxError(s, ykp1, ykp1, err);
+        if (count > 0) {
+            double diver = maxErr / dmax(lastErr, TINY);
+            if (diver > 2 && count > 5) {
+                *conv = 0;
+                return 1000.0;
+            }
+        }
+        lastErr = maxErr;
+        count++;
+        if (isnan(maxErr)) {
+            *conv = 0;
+            return 1000.0;
+        }
+        if (count > 1000) {
+            *conv = 0;
+            return maxErr;
+        }
+    } while (maxErr > 1.0);
+    *conv = 1;
+    return maxErr;
+}
+
+/* BDF23::solve(t0, y0, dydt0, dt, y), BDF23.cpp:176-268 (order 2): the first
+ * iOrder + 1 steps of a solve estimate y - y0 - dt f(y); later ones the
+ * third difference of the step-start states, (dt_n + dt_n-1)/6 */
+static double bdf_step(ie_ws *s, const double *y0, double dt, double *y) {
+    const int n = s->n;
+    double yh[NMAX], e[NMAX];
+    for (int i = 0; i < n; ++i) {
+        double v = 0.0;
+        for (int k = s->nalpha - 1; k > 0; --k) v += s->alpha[k] * s->ykl[k - 1][i];
+        yh[i] = v;
+    }
+    memcpy(s->yk, y0, sizeof(double) * n);
+    int conv = 1;
+    double errN = bdf_newton(s, yh, dt, s->yk, &conv);
+    memcpy(y, s->yk, sizeof(double) * n);
+    if (!conv) return errN;
+    if (s->countStep <= 2) {
+        orc_dydt(s->m, s->o->flow_type, s->pr, s->yk, s->f);
+        s->rhs++;
+        for (int i = 0; i < n; ++i) e[i] = s->yk[i] - y0[i] - dt * s->f[i];
+        return maxIncrement(s, y0, s->yk, e);
+    }
+    const double dtn = s->lts4[0], dtn1 = s->lts4[1], dtn2 = s->lts4[2];
+    const double f1 = 1.0 / dtn;
+    const double f2 = (1.0 / dtn1) * (1.0 + dtn / dtn1);
+    const double f3 = dtn / (dtn1 * dtn2);
+    const double ltef = (dtn + dtn1) / 6.0;
+    for (int i = 0; i < n; ++i) {
+        e[i] = f1 * (s->yk[i] - s->ykl[0][i]) - f2 * (s->ykl[0][i] - s->ykl[1][i]) +
+               f3 * (s->ykl[1][i] - s->ykl[2][i]);
+        e[i] *= ltef;
+    }
+    return maxIncrement(s, y0, s->yk, e);
+}
+
+/* BDF23::adaptiveSolve, BDF23.cpp:391-437: an attempt whose error grew from
+ * the previous attempt's but stays below 5 is accepted as it is */
+static int bdf_adaptive(ie_ws *s, int iCorder, double *t, double *dt0, double *y) {
+    const ohc_opts *o = s->o;
+    double dt = *dt0, error = 0.0, lastError = LARGE;
+    s->lts4[3] = s->lts4[2];
+    s->lts4[2] = s->lts4[1];
+    s->lts4[1] = s->lts4[0];
+    int count = 0;
+    do {
+        s->lts4[0] = dt;
+        bdf_set_alpha(s, iCorder);
+        s->attempts++;
+        error = bdf_step(s, y, dt, s->yTemp);
+        if (!checkNewY(s, &dt, y, s->yTemp) && count < 100) {
+            count++;
+            error = 2;
+            if (dt <= VERYTINY) return OHC_ERR_NUMERIC;
+            continue;
+        }
+        if (lastError < error && error < 5) break;
+        if (error > 1.0) {
+            double scale = dmax(o->safe_scale * pow(error, -o->alpha_inc), o->min_scale);
+            dt *= scale;
+            if (dt <= VERYTINY) return OHC_ERR_NUMERIC;
+        }
+        lastError = error;
+    } while (error > 1.0);
+    *t += dt;
+    memcpy(y, s->yTemp, sizeof(double) * s->n);
+    if (error > pow(o->max_scale / o->safe_scale, -1.0 / o->alpha_inc))
+        *dt0 = dmin(dmax(o->safe_scale * pow(error, -o->alpha_inc), o->min_scale), o->max_scale) *
+               dt;
+    else
+        *dt0 = dmax(o->safe_scale * o->max_scale * dt, dt);
+    s->steps++;
+    s->cell_steps++;
+    return 0;
+}
+
+/* BDF23::solve(t0, tn, dt0, y), BDF23.cpp:439-509: first step
+ * min(0.1 tn, 1e-8) at order 1, then order 2; the step doubles on steps 4
+ * and 5; the history shifts before each step */
+static int bdf_solve(ie_ws *s, double *tn, double *dt0, double *y) {
+    const int n_ = s->n;
+    const double t0 = 0.0;
+    double t = t0, dt = *dt0, dtReached = t0;
+    int last = 0;
+    for (int n = 0; n < s->o->max_steps; ++n) {
+        s->countStep = n;
+        int iCorder;
+        if (n == 0) {
+            dt = dmin(0.1 * (*tn), 1e-8);
+            iCorder = 1;
+            memcpy(s->ykl[0], y, sizeof(double) * n_);
+        } else if (n == 1) {
+            iCorder = 2;
+            memcpy(s->ykl[1], s->ykl[0], sizeof(double) * n_);
+            memcpy(s->ykl[0], y, sizeof(double) * n_);
+        } else {
+            iCorder = 2;
+            if (n <= 3) {
+                memcpy(s->ykl[2], s->ykl[1], sizeof(double) * n_);
+            } else {
+                if (n <= 5) dt *= 2.0;
+                memcpy(s->ykl[3], s->ykl[2], sizeof(double) * n_);
+                memcpy(s->ykl[2], s->ykl[1], sizeof(double) * n_);
+            }
+            memcpy(s->ykl[1], s->ykl[0], sizeof(double) * n_);
+            memcpy(s->ykl[0], y, sizeof(double) * n_);
+        }
+        double dtTry0 = dt;
+        if ((t + dt - *tn) * (t + dt - t0) > 0) {
+            dt = *tn - t;
+            last = 1;
+        } else if (t + dt == *tn) {
+            last = 1;
+        }
+        int rc = bdf_adaptive(s, iCorder, &t, &dt, y);
+        if (rc) return rc;
+        dtReached = t;
+        if ((t - *tn) * (*tn - t0) >= 0) {
+            if (n > 0 && last) *dt0 = dmin(dtTry0, *tn);
+            return 0;
+        }
+    }
+    *tn = dtReached;
+    return 0;
+}
+
 /* Per-cell driver: chemistrySource::solve(dt, f) body (:1513-1554) and
  * solve(t, dT, p, T, rho, yi) (:1300-1325).  mode 0: R output, 1: update. */
 static int cell_integrate(ie_ws *s, double T, double p, double rho, double leftTime,
@@ -1124,7 +1321,9 @@ static int cell_integrate(ie_ws *s, double T, double p, double rho, double leftT
         /* ODESolver::solve, ODESolver.inl:74-90 */
         for (int i = 0; i < ns; ++i) y[i] = c[i];
         y[ns] = Tc;
-        int rc = s->o->integrator == 2 ? tr_solve(s, &dtt, subDt, y) : ie_solve(s, &dtt, subDt, y);
+        int rc = s->o->integrator == 2   ? tr_solve(s, &dtt, subDt, y)
+                 : s->o->integrator == 3 ? bdf_solve(s, &dtt, subDt, y)
+                                         : ie_solve(s, &dtt, subDt, y);
         if (rc) return rc;
         Tc = y[ns];
         for (int i = 0; i < ns; ++i) c[i] = y[i];

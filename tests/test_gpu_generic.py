@@ -270,3 +270,82 @@ def test_generic_trapezoidal_matches_oracle(which, flow, h2mech, h2oracle, syn32
     _check_states(T, Y, To, Yo)
     assert abs(s["steps"] - s_o["steps"]) <= 0.01 * s_o["steps"]
     assert abs(s["newton_iters"] - s_o["newton_iters"]) <= 0.02 * s_o["newton_iters"]
+
+
+def _bdf23_case(which, flow, dt, h2mech, h2oracle, syn32, syn32_oracle, jac=1):
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource
+    from openhurricane_amd.sampler import h2air_states, synthetic_workload
+
+    if which == "h2air":
+        mech, orc = h2mech, h2oracle
+        o = _lib.default_opts(atol=1e-10, rtol=1e-6, atol_inc=1e-8, rtol_inc=1e-3, integrator=3,
+                              flow_type=flow, kernel=2)
+        n = 512
+        st = h2air_states(mech.species, mech.tables()["W"], n, seed=23)
+    else:
+        mech, orc = syn32, syn32_oracle
+        o = _opts(integrator=3, jacobian=jac, atol_inc=1e-8, rtol_inc=1e-3, flow_type=flow)
+        n = 96
+        st = synthetic_workload(mech.species, mech.tables()["W"], n, seed=29)
+    cs = ChemistrySource(mech, opts=o)
+    assert cs.kernel_name.startswith("wave:"), cs.kernel_name
+    dtv = np.full(n, dt)
+    To, Yo, sub_o, s_o = orc.solve_update(o, st["T"], st["p"], st["rho"], st["Y"], dtv, 1.0,
+                                          nthreads=16)
+    T, Y = st["T"].copy(), st["Y"].copy()
+    sub, s = cs.solve_update(T, st["p"], st["rho"], Y, dtv, 1.0)
+    print("gpu", s, "oracle", s_o)
+    return T, Y, s, To, Yo, s_o
+
+
+@pytest.mark.parametrize("flow,jac", [(0, 1), (1, 1), (0, 0)])
+def test_generic_bdf23_matches_oracle(flow, jac, h2mech, h2oracle, syn32, syn32_oracle):
+    """BDF23 (integrator = 3, order 2) on the generic kernels against the
+    oracle's restatement, at the reference's BDF23 step tolerances
+    (atol 1e-8, rtol 1e-3), on the synthetic 32-species mechanism over 1 us:
+    states, step and Newton counts.  Here the algorithm is well conditioned:
+    perturbing the oracle's input temperatures by 1e-14 moves no final
+    temperature by more than 3.3e-13 with the reference's Jacobian
+    (jacobian = 1) and 3.4e-8 with the exact one, and the step counts by at
+    most 5 of 8506."""
+    T, Y, s, To, Yo, s_o = _bdf23_case("syn32", flow, 1e-6, h2mech, h2oracle, syn32,
+                                       syn32_oracle, jac)
+    _check_states(T, Y, To, Yo)
+    assert abs(s["steps"] - s_o["steps"]) <= 0.01 * s_o["steps"]
+    assert abs(s["newton_iters"] - s_o["newton_iters"]) <= 0.02 * s_o["newton_iters"]
+
+
+@pytest.mark.parametrize("dt", [1e-7, 1e-6])
+def test_generic_bdf23_sensitive_regime(dt, h2mech, h2oracle, syn32, syn32_oracle):
+    """H2-air at the BDF23 tolerances is a regime where the reference's step
+    control is itself chaotic: its error estimate stops falling with the step
+    size, and whether an attempt is accepted ("error grew from the last
+    attempt's and is below 5") turns on rounding.  On the oracle, perturbing
+    the input temperatures of these 512 cells by 1e-15 .. 1e-13 changes up
+    to 7 cells by up to 8.6e-5 in Y over 0.1 us, and over 1 us changes 76
+    cells by up to 1.1e-4 in T and 5e-3 in Y and makes one fail with a
+    step-size underflow.  The GPU run must stay within that envelope: most
+    cells identical, the others within the step tolerance, step counts
+    within 2 %."""
+    T, Y, s, To, Yo, s_o = _bdf23_case("h2air", 0, dt, h2mech, h2oracle, syn32, syn32_oracle)
+    dT = np.abs(T - To) / To
+    print("cells off by > 1e-9:", int((dT > 1e-9).sum()), "max", dT.max())
+    assert (dT > 1e-9).sum() <= 128
+    assert dT.max() <= 1e-3
+    assert np.all(np.abs(Y - Yo) <= 1e-2 * np.maximum(np.abs(Yo), 1e-3))
+    assert abs(s["steps"] - s_o["steps"]) <= 0.02 * s_o["steps"]
+
+
+def test_bdf23_options_are_checked(h2mech):
+    """BDF23 runs on the generic kernels in FP64 only: the lane kernel and the
+    FP32 split are rejected at context creation."""
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource
+
+    with pytest.raises(_lib.OhcError):
+        ChemistrySource(h2mech, opts=_lib.default_opts(integrator=3, kernel=1))
+    with pytest.raises(_lib.OhcError):
+        ChemistrySource(h2mech, opts=_lib.default_opts(integrator=3, precision=1))
+    cs = ChemistrySource(h2mech, opts=_lib.default_opts(integrator=3))
+    assert cs.kernel_name.startswith("wave:"), cs.kernel_name

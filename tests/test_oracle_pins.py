@@ -274,3 +274,45 @@ def test_trapezoidal_restatement_converges_at_second_order(h2mech, h2oracle):
     assert e[2] < 3e-4, e
     # fewer steps than backward Euler for the same step tolerance
     assert steps[(2, 1e-8)] < steps[(0, 1e-8)]
+
+
+def test_bdf23_restatement(h2mech, h2oracle):
+    """BDF23 (integrator = 3, order 2): approaches the tight Radau solution as
+    the step tolerance tightens within the range where the reference's
+    controller works, is at least as accurate as ImplicitEuler at the same
+    tolerance, and -- as the reference does -- gives up with a step-size
+    underflow at tolerances too tight for its error estimate."""
+    from scipy.integrate import solve_ivp
+
+    from openhurricane_amd import _lib
+    from openhurricane_amd.sampler import h2air_states
+
+    W = h2mech.tables()["W"]
+    n, dt = 6, 1e-6
+    st = h2air_states(h2mech.species, W, n, seed=11, T_range=(1600, 2500))
+    refT, refY = np.zeros(n), np.zeros((9, n))
+    for i in range(n):
+        y0 = np.concatenate([st["rho"][i] * st["Y"][:, i] / W, [st["T"][i]]])
+        p = st["p"][i]
+        sol = solve_ivp(lambda t, y: h2oracle.dydt(y, p), (0, dt), y0, method="Radau",
+                        rtol=1e-12, atol=1e-22,
+                        jac=lambda t, y: h2oracle.jacobian(y, p, exact_dkp=True)[1])
+        y = sol.y[:, -1]
+        refT[i] = y[-1]
+        refY[:, i] = y[:-1] * W / (y[:-1] * W).sum()
+
+    def err(integ, rtol):
+        o = _lib.default_opts(atol=1e-16, rtol=rtol, atol_inc=1e-16, rtol_inc=rtol,
+                              integrator=integ)
+        T, Y, _, _ = h2oracle.solve_update(o, st["T"], st["p"], st["rho"], st["Y"],
+                                           np.full(n, dt), 1.0, subDt=np.full(n, 1e-13))
+        return max(np.max(np.abs(T - refT) / refT), np.max(np.abs(Y - refY) / np.maximum(refY, 1e-3)))
+
+    e3, e4 = err(3, 1e-3), err(3, 1e-4)
+    print("BDF23", e3, e4, "ImplicitEuler", err(0, 1e-4))
+    assert e3 > 2 * e4 and e4 < 0.02
+    assert e4 <= 1.1 * err(0, 1e-4)
+    o = _lib.default_opts(atol=1e-16, rtol=1e-8, atol_inc=1e-16, rtol_inc=1e-8, integrator=3)
+    with pytest.raises(RuntimeError):
+        h2oracle.solve_update(o, st["T"], st["p"], st["rho"], st["Y"], np.full(n, dt), 1.0,
+                              subDt=np.full(n, 1e-13))
