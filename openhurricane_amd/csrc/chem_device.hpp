@@ -104,11 +104,15 @@ struct Thermo {
 
 // Coefficient set of species k at T (JANAF.inl:34-40) -- a[0..6] and the
 // host-precomputed quotients a_i/5.. (h), a_i/4.. (s), a_i/20.. (g).
-template <class M, int k>
+// R = -1 picks the set per lane (T < Tcommon: low); R = 0 / 1 fixes the low /
+// high set, for code where every lane of the wavefront is known to be on that
+// side, so the coefficients stay compile-time constants instead of per-lane
+// selects (two moves and two selects per FP64 value).
+template <class M, int k, int R = -1>
 struct Coef {
     double a[7], h4, h3, h2, h1, s4, s3, s2, g4, g3, g2, g1;
     __device__ __forceinline__ explicit Coef(double T) {
-        const bool lo = T < M::Tcommon[k];
+        const bool lo = R < 0 ? T < M::Tcommon[k] : R == 0;
 #pragma unroll
         for (int c = 0; c < 7; ++c) a[c] = lo ? M::alo[k][c] : M::ahi[k][c];
         h4 = lo ? M::hlo[k][0] : M::hhi[k][0];
@@ -125,9 +129,32 @@ struct Coef {
     }
 };
 
+// Lowest and highest Tcommon of the mechanism.
+template <class M>
+struct TcommonRange {
+    static constexpr double lo() {
+        double v = M::Tcommon[0];
+        for (int k = 1; k < M::NS; ++k) v = M::Tcommon[k] < v ? M::Tcommon[k] : v;
+        return v;
+    }
+    static constexpr double hi() {
+        double v = M::Tcommon[0];
+        for (int k = 1; k < M::NS; ++k) v = M::Tcommon[k] > v ? M::Tcommon[k] : v;
+        return v;
+    }
+};
+
+// Whether every active lane of the wavefront is at or above every Tcommon
+// (the high coefficient sets apply throughout).
+template <class M>
+__device__ __forceinline__ bool wave_all_high(double T) {
+    constexpr double Th = TcommonRange<M>::hi();
+    return __all(T >= Th);
+}
+
 // cp0 with the JANAF temperature clamp (JANAF.cpp:155-159), per unit mass.
-template <class M, int k>
-__device__ __forceinline__ double janaf_cp0(const Coef<M, k> &c, double T) {
+template <class M, int k, int R>
+__device__ __forceinline__ double janaf_cp0(const Coef<M, k, R> &c, double T) {
     const double Tl = M::Tlow[k], Th = M::Thigh[k];
     const double T0 = (T < Tl || T > Th) ? fmin(fmax(T, Tl), Th) : T;
     const double *a = c.a;
@@ -135,12 +162,11 @@ __device__ __forceinline__ double janaf_cp0(const Coef<M, k> &c, double T) {
 }
 
 // Molar G and dG/dT of species k (JANAF.cpp:199-212, thermo.inl:195-205).
-template <class M, int k>
-__device__ __forceinline__ void species_G(double T, double lnT, double invTm, double &G,
-                                          double &dG) {
-    const Coef<M, k> c(T);
+template <class M, int k, int R>
+__device__ __forceinline__ void species_G(const Coef<M, k, R> &c, double T, double lnT,
+                                          double invTm, double &G, double &dG) {
     const double *a = c.a;
-    const double cp0 = janaf_cp0<M, k>(c, T);
+    const double cp0 = janaf_cp0(c, T);
     const double s0 = ((((c.s4 * T + c.s3) * T + c.s2) * T + a[1]) * T + a[0] * lnT + a[6]);
     const double g0 =
         (((((-c.g4 * T - c.g3) * T - c.g2) * T - c.g1) * T + a[0] * (1.0 - lnT) - a[6]) * T + a[5]);
@@ -150,12 +176,18 @@ __device__ __forceinline__ void species_G(double T, double lnT, double invTm, do
     dG = (cp0 - (s0 + T * ds0)) * W;
 }
 
-// Molar Cp, H and dCp/dT of species k (JANAF.cpp:155-189, :214-217).
 template <class M, int k>
-__device__ __forceinline__ void species_CH(double T, double &Cp, double &H, double &dCp) {
-    const Coef<M, k> c(T);
+__device__ __forceinline__ void species_G(double T, double lnT, double invTm, double &G,
+                                          double &dG) {
+    species_G<M, k>(Coef<M, k>(T), T, lnT, invTm, G, dG);
+}
+
+// Molar Cp, H and dCp/dT of species k (JANAF.cpp:155-189, :214-217).
+template <class M, int k, int R>
+__device__ __forceinline__ void species_CH(const Coef<M, k, R> &c, double T, double &Cp, double &H,
+                                           double &dCp) {
     const double *a = c.a;
-    const double cp0 = janaf_cp0<M, k>(c, T);
+    const double cp0 = janaf_cp0(c, T);
     const double Tl = M::Tlow[k], Th = M::Thigh[k];
     const bool out = (T < Tl) || (T > Th);
     const double Tx = out ? (T < Tl ? Tl : Th) : T;
@@ -166,6 +198,11 @@ __device__ __forceinline__ void species_CH(double T, double &Cp, double &H, doub
     Cp = cp0 * W;
     H = h * W;
     dCp = dcp * W;
+}
+
+template <class M, int k>
+__device__ __forceinline__ void species_CH(double T, double &Cp, double &H, double &dCp) {
+    species_CH<M, k>(Coef<M, k>(T), T, Cp, H, dCp);
 }
 
 // species_G evaluated in FP32 (the split-precision mode's Gibbs terms).
@@ -513,7 +550,9 @@ __device__ __forceinline__ FallOff<M, r> falloff(const TFac &tf, double Mtb) {
 // F = float is the split-precision mode: Gibbs energies, equilibrium and
 // Arrhenius rate constants in FP32, everything else (concentration products,
 // net rates, Jacobian, temperature equation) in FP64.
-template <class M, bool WITH_J, int FLOW, class F = double>
+// UNI: when every lane of the wavefront is above every Tcommon, evaluate the
+// thermo with the high coefficient sets as constants (else per-lane selects).
+template <class M, bool WITH_J, int FLOW, class F = double, bool UNI = false>
 __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
                                         double (&f)[M::NS + 1],
                                         double (&J)[M::NS + 1][M::NS + 1], bool exact = false) {
@@ -521,17 +560,25 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
     constexpr bool fp32 = std::is_same_v<F, float>;
     const double T = y[NS];
     const TFac tf = tfac(T);
+    const bool hi = UNI && wave_all_high<M>(T);
     Thermo<M> th;  // G / dG now; Cp / H / dCp after the reaction loop
-    sfor<0, NS>([&](auto k) {
-        if constexpr (fp32) species_G32<M, k>(T, (float)tf.lnT, (float)tf.invTm, th.G[k], th.dG[k]);
-        else species_G<M, k>(T, tf.lnT, tf.invTm, th.G[k], th.dG[k]);
-        if constexpr (M::KC_SPECIES && !fp32) {
-            const double gRT = th.G[k] * tf.invRuT;
-            th.dG[k] = (th.dG[k] - th.G[k] * tf.invTm) * tf.invRuT;
-            th.G[k] = exp(-gRT);
-            th.H[k] = frcp(th.G[k]);  // exp(+G/(Ru T)), slot reused until the T row
-        }
-    });
+    auto gibbs = [&](auto Rc) {
+        constexpr int R = decltype(Rc)::value;
+        sfor<0, NS>([&](auto k) {
+            if constexpr (fp32)
+                species_G32<M, k>(T, (float)tf.lnT, (float)tf.invTm, th.G[k], th.dG[k]);
+            else
+                species_G<M, k>(Coef<M, k, R>(T), T, tf.lnT, tf.invTm, th.G[k], th.dG[k]);
+            if constexpr (M::KC_SPECIES && !fp32) {
+                const double gRT = th.G[k] * tf.invRuT;
+                th.dG[k] = (th.dG[k] - th.G[k] * tf.invTm) * tf.invRuT;
+                th.G[k] = exp(-gRT);
+                th.H[k] = frcp(th.G[k]);  // exp(+G/(Ru T)), slot reused until the T row
+            }
+        });
+    };
+    if (hi) gibbs(std::integral_constant<int, 1>{});
+    else gibbs(std::integral_constant<int, -1>{});
     double ci[NS];
     sfor<0, NS>([&](auto k) { ci[k] = fmax(y[k], 0.0); });
     // third-body concentrations and the per-group row weights of dln(M)/dc
@@ -679,7 +726,14 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
         });
     }
     // ---- temperature row (chemistrySource.cpp:993-1020 / :1021-1054)
-    sfor<0, NS>([&](auto k) { species_CH<M, k>(T, th.Cp[k], th.H[k], th.dCp[k]); });
+    auto cph = [&](auto Rc) {
+        constexpr int R = decltype(Rc)::value;
+        sfor<0, NS>([&](auto k) {
+            species_CH<M, k>(Coef<M, k, R>(T), T, th.Cp[k], th.H[k], th.dCp[k]);
+        });
+    };
+    if (hi) cph(std::integral_constant<int, 1>{});
+    else cph(std::integral_constant<int, -1>{});
     double Hx[NS], Cx[NS];
     if constexpr (FLOW == 0) {
         sfor<0, NS>([&](auto k) {
@@ -731,14 +785,19 @@ __device__ __forceinline__ void lu_solve(double (&A)[N][N], double (&b)[N]) {
             sfor<0, k>([&](auto t) { sum += A[i][t] * A[t][k]; });
             s[i] = A[i][k] - sum;
         });
-        int ik = k;
-        double maxs = s[k];
-        sfor<k + 1, N>([&](auto i) {
-            const bool g = fabs(s[i]) > fabs(maxs);
-            ik = g ? (int)i : ik;
-            maxs = g ? s[i] : maxs;
-        });
-        if (__any(ik != k)) {
+        // a row swap is needed where some |s_i| (i > k) exceeds |s_k|; only
+        // then is the pivot row located (the first such maximum, as in the
+        // sequential search)
+        double mo = 0.0;
+        sfor<k + 1, N>([&](auto i) { mo = fmax(mo, fabs(s[i])); });
+        if (__any(mo > fabs(s[k]))) {
+            int ik = k;
+            double maxs = s[k];
+            sfor<k + 1, N>([&](auto i) {
+                const bool g = fabs(s[i]) > fabs(maxs);
+                ik = g ? (int)i : ik;
+                maxs = g ? s[i] : maxs;
+            });
             sfor<k + 1, N>([&](auto i) {
                 const bool sw = (ik == (int)i);
                 if (__any(sw)) {

@@ -358,7 +358,7 @@ __global__ __launch_bounds__(64) void ie_lane_kernel(const LaneParams P) {
             double yk[N];
 #pragma unroll
             for (int i = 0; i < N; ++i) yk[i] = YK(i);
-            eval_fJ<M, true, FLOW, F>(yk, pr, f, J, P.exact_dkp != 0);
+            eval_fJ<M, true, FLOW, F, true>(yk, pr, f, J, P.exact_dkp != 0);
         }
 
         bool have = false;
