@@ -206,10 +206,10 @@ __device__ __forceinline__ void species_CH(double T, double &Cp, double &H, doub
 }
 
 // species_G evaluated in FP32 (the split-precision mode's Gibbs terms).
-template <class M, int k>
+template <class M, int k, int R = -1>
 __device__ __forceinline__ void species_G32(double Td, float lnT, float invTm, double &G,
                                             double &dG) {
-    const bool lo = Td < M::Tcommon[k];
+    const bool lo = R < 0 ? Td < M::Tcommon[k] : R == 0;
     const float T = (float)Td;
     float a[7];
 #pragma unroll
@@ -566,7 +566,7 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
         constexpr int R = decltype(Rc)::value;
         sfor<0, NS>([&](auto k) {
             if constexpr (fp32)
-                species_G32<M, k>(T, (float)tf.lnT, (float)tf.invTm, th.G[k], th.dG[k]);
+                species_G32<M, k, R>(T, (float)tf.lnT, (float)tf.invTm, th.G[k], th.dG[k]);
             else
                 species_G<M, k>(Coef<M, k, R>(T), T, tf.lnT, tf.invTm, th.G[k], th.dG[k]);
             if constexpr (M::KC_SPECIES && !fp32) {
