@@ -452,14 +452,16 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
 template <int N>
 struct WaveLDS {
     // row stride of the Jacobian: odd, so that a lane-per-row access pattern
-    // (lane i reading row i) hits distinct LDS banks
-    static constexpr int JS = (N + 1) | 1;
+    // (lane i reading row i) hits distinct LDS banks.  Every array is sized
+    // for the N unknowns (not the 64 lanes) so that more wavefronts fit in
+    // a CU's LDS.
+    static constexpr int JS = N | 1;
     double J[N * JS];
-    double c[64], G[64], dG[64], Cp[64], H[64], dCp[64];
-    double w[64];     // net production rates omega_k
+    double c[N], G[N], dG[N], Cp[N], H[N], dCp[N];  // species k in [k]
+    double w[N];      // net production rates omega_k
     double gam[64];   // third-body / fall-off coefficient of each reaction lane of a chunk
-    double dinv[64];  // 1 / pivot of each elimination step
-    int perm[64];     // pivot lane of each elimination step
+    double dinv[N];   // 1 / pivot of each elimination step
+    int perm[N];      // pivot lane of each elimination step
 };
 
 // dy/dt (and the Jacobian into L.J when WITH_J) at the wavefront-distributed
@@ -481,7 +483,7 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
     }
     if (WITH_J)
         for (int e = lane; e < N * WaveLDS<N>::JS; e += 64) L.J[e] = 0.0;
-    L.w[lane] = 0.0;
+    if (lane < N) L.w[lane] = 0.0;
     __syncthreads();
     for (int base = 0; base < M.nr; base += 64) {
         const int r = base + lane;
