@@ -129,26 +129,19 @@ struct Coef {
     }
 };
 
-// Lowest and highest Tcommon of the mechanism.
+// Highest Tcommon of the mechanism.
 template <class M>
-struct TcommonRange {
-    static constexpr double lo() {
-        double v = M::Tcommon[0];
-        for (int k = 1; k < M::NS; ++k) v = M::Tcommon[k] < v ? M::Tcommon[k] : v;
-        return v;
-    }
-    static constexpr double hi() {
-        double v = M::Tcommon[0];
-        for (int k = 1; k < M::NS; ++k) v = M::Tcommon[k] > v ? M::Tcommon[k] : v;
-        return v;
-    }
-};
+constexpr double tcommon_max() {
+    double v = M::Tcommon[0];
+    for (int k = 1; k < M::NS; ++k) v = M::Tcommon[k] > v ? M::Tcommon[k] : v;
+    return v;
+}
 
 // Whether every active lane of the wavefront is at or above every Tcommon
 // (the high coefficient sets apply throughout).
 template <class M>
 __device__ __forceinline__ bool wave_all_high(double T) {
-    constexpr double Th = TcommonRange<M>::hi();
+    constexpr double Th = tcommon_max<M>();
     return __all(T >= Th);
 }
 

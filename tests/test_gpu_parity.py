@@ -169,6 +169,44 @@ def test_longest_first_schedule_is_bitwise_neutral(h2mech, c1opts, states):
         assert s["newton_iters"] == outs[0][3]["newton_iters"]
 
 
+def test_wave_uniform_thermo_is_bitwise_neutral(h2mech, c1opts):
+    """When every lane of a wavefront is above Tcommon the thermo runs with the
+    high NASA sets as constants, otherwise with a per-lane choice.  The same
+    hot cells integrated alone (uniform wavefronts) and with a cold cell
+    (T < Tcommon) placed in every 64-cell group (mixed wavefronts) must give
+    identical bits."""
+    from openhurricane_amd.chemistry import ChemistrySource
+    from openhurricane_amd.sampler import h2air_states
+
+    t = h2mech.tables()
+    hot = h2air_states(h2mech.species, t["W"], 2048, seed=31, T_range=(1100.0, 2500.0))
+    cold = h2air_states(h2mech.species, t["W"], 2048 // 63 + 1, seed=32, T_range=(900.0, 990.0))
+    # mixed layout: cell 64 g is cold, cells 64 g + 1 .. 64 g + 63 are hot
+    n_mix = 2048 + len(cold["T"])
+    is_cold = np.zeros(n_mix, dtype=bool)
+    is_cold[::64] = True
+    is_cold = is_cold[:n_mix]
+    assert is_cold.sum() == len(cold["T"]) and (~is_cold).sum() == 2048
+    mix = {}
+    for k in ("T", "p", "rho"):
+        v = np.empty(n_mix)
+        v[is_cold] = cold[k][: is_cold.sum()]
+        v[~is_cold] = hot[k][: (~is_cold).sum()]
+        mix[k] = v
+    Ym = np.empty((hot["Y"].shape[0], n_mix))
+    Ym[:, is_cold] = cold["Y"][:, : is_cold.sum()]
+    Ym[:, ~is_cold] = hot["Y"][:, : (~is_cold).sum()]
+    nh = int((~is_cold).sum())
+    cs = ChemistrySource(h2mech, opts=c1opts)
+    T1, Y1 = hot["T"][:nh].copy(), np.ascontiguousarray(hot["Y"][:, :nh])
+    sub1, _ = cs.solve_update(T1, hot["p"][:nh], hot["rho"][:nh], Y1, np.full(nh, 1e-6), 1.0)
+    T2, Y2 = mix["T"].copy(), np.ascontiguousarray(Ym)
+    sub2, _ = cs.solve_update(T2, mix["p"], mix["rho"], Y2, np.full(n_mix, 1e-6), 1.0)
+    np.testing.assert_array_equal(T2[~is_cold], T1)
+    np.testing.assert_array_equal(Y2[:, ~is_cold], Y1)
+    np.testing.assert_array_equal(sub2[~is_cold], sub1)
+
+
 @pytest.mark.parametrize("without_last", [0, 1])
 def test_source_terms_match_oracle(cs, h2oracle, states, c1opts, without_last):
     """calculateSourceTerms / calculateSourceTermsImp: R_k and the (coupled)
