@@ -13,7 +13,9 @@
  *   omega / RHS   mass and element conservation, dh/dt = 0 at constant p
  *   Jacobian      central finite differences of the RHS
  *   LU            numpy.linalg.solve, pivot-requiring matrices
- *   integrator    scipy Radau at rtol 1e-12 (accuracy), conservation
+ *   integrator    scipy Radau at rtol 1e-12 (accuracy), conservation; the
+ *                 trapezoidal rule at second order; BDF23 between step
+ *                 tolerances 1e-3 and 1e-4 (and its underflow at 1e-8)
  *   T(h)          round trip h(T(h)) = h and limiter flags
  *   transport     single-species and binary closed forms, symmetry
  */
