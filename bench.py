@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--workload", default="h2air", choices=["h2air", "syn32", "syn54"],
                     help="h2air: 9-species H2-air (default); syn32 / syn54: synthetic "
                          "32-species/150-reaction or 54/270 mechanisms on the generic kernel")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the compact gated-surrogate leg (about a minute)")
     ap.add_argument("--no-split", action="store_true",
                     help="skip the FP32 split-precision comparison")
     ap.add_argument("--jacobian", type=int, default=0,
@@ -166,6 +168,7 @@ def main():
             "steps": stats["steps"], "flops_per_newton": integrator_flops(tab)["F_newton"]}
     split = None if args.no_split else split_precision(args, mech, g, T0, Y0, dt, stream)
     others = None if args.no_split else other_integrators(mech, g, T0, Y0, dt, stream)
+    surr = None if args.no_surrogate else surrogate_leg()
     cpu = None
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
@@ -182,9 +185,45 @@ def main():
                    "cells": n, "mechanism": "h2air", "integrator": "ImplicitEuler",
                    "parallelism": "single GPU"},
         "roofline": roof, "cpu_baseline": cpu, "split_precision": split,
-        "other_integrators": others,
+        "other_integrators": others, "surrogate": surr,
     }
     print(json.dumps(line), flush=True)
+
+
+SURR_PAIRS, SURR_EVAL, SURR_EPOCHS = 1 << 22, 1 << 20, 10
+
+
+def surrogate_leg():
+    """Gated surrogate + exact fallback against the plain FP64 integrator, at a
+    compact scale of the 32-species surrogate workload (the full-scale run is
+    `python -m openhurricane_amd.surrogate_workflow`): two 34->256^3->33 BF16
+    networks trained on the GPU on pairs the GPU integrator generates, then
+    timed and checked on held-out cells of the same distribution.  A failure
+    here is reported in the line, never raised."""
+    try:
+        from openhurricane_amd.surrogate_workflow import run
+
+        r = run(["--pairs", str(SURR_PAIRS), "--eval", str(SURR_EVAL), "--epochs", str(SURR_EPOCHS),
+                 "--peer-tol", "1e-5", "1e-4"])
+    except Exception as e:  # the plain-path line must still be printed
+        return {"error": f"{type(e).__name__}: {e}"}
+    gated = []
+    for c in r["gated"]:
+        gated.append({"gate": ("element change <= %g" % c["gate_tol"] if c["gate_tol"] < 1e29
+                               else "none") + ("" if c.get("peer_tol") is None else
+                                               ", peer agreement %g" % c["peer_tol"])
+                      + (", FP8" if c.get("precision") else ""),
+                      "cells_per_s": c["cells_per_s"], "speedup_vs_plain": c["speedup"],
+                      "fallback_frac": c["fallback_frac"], "max_rel_err": max(
+                          c["max_rel_err_T"], c["max_rel_err_Ymajor"]),
+                      "max_rel_err_T": c["max_rel_err_T"]})
+    return {"workload": f"synthetic 32sp/150rxn, {SURR_PAIRS:,} training pairs from the GPU "
+                        f"integrator, {SURR_EVAL:,} held-out cells (20% ignition front), dt 1e-6 s; "
+                        f"two 34->256^3->33 GELU networks, BF16 on MFMA, {SURR_EPOCHS} epochs",
+            "plain_cells_per_s": r["plain"]["cells_per_s"],
+            "network_alone_cells_per_s": r["inference_bf16"]["cells_per_s"],
+            "seconds_train": r["seconds_train"], "gated": gated,
+            "target": ">= 5x the plain path at <= 1e-3 max relative error (T, Y >= 1e-3)"}
 
 
 def autotune(cs, g, dt, m=1 << 20):

@@ -185,7 +185,8 @@ def evaluate(mech, cs, nets, n, seed, front, configs, dt=1e-6, steps=2):
     return out
 
 
-def main(argv=None):
+def run(argv=None):
+    """Generate pairs, train, evaluate; returns the result dictionary."""
     ap = argparse.ArgumentParser()
     ap.add_argument("--mech", default="syn32", choices=["syn32", "syn54"])
     ap.add_argument("--pairs", type=int, default=1 << 24)
@@ -244,7 +245,11 @@ def main(argv=None):
                 {"dT": args.delta_T, "dlnY": args.delta_Y},
                 "loss_history": hist, "seconds_generate": t_gen, "seconds_train": t_fit,
                 "train_samples_per_s": len(nets) * args.pairs * args.epochs / t_fit})
-    print(json.dumps(res), flush=True)
+    return res
+
+
+def main(argv=None):
+    print(json.dumps(run(argv)), flush=True)
 
 
 if __name__ == "__main__":
