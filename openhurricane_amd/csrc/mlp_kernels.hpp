@@ -63,6 +63,10 @@ struct GemmArgs {
     int64_t ldc;
     const float *bias;  // [N]
     const float *Z;     // EPI_GELU_BWD: saved pre-activation, ld ldc
+    // split-K (EPI_F32 only): block z reduces k in [z kc, min(K, (z+1) kc))
+    // into C32 + z zstride; splitk_sum then adds the partial tiles in order
+    int kc;
+    int64_t zstride;
 };
 
 constexpr int kTM = 64, kTN = 64, kTK = 32, kLdsK = kTK + 8;  // +8: LDS row padding
@@ -77,8 +81,9 @@ __global__ __launch_bounds__(256) void gemm_bf16(const GemmArgs g) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int kb = (int)blockIdx.z * g.kc, ke = min(g.K, kb + g.kc);
     f32x16 acc = {};
-    for (int k0 = 0; k0 < g.K; k0 += kTK) {
+    for (int k0 = kb; k0 < ke; k0 += kTK) {
         // ---- stage A and B slices (256 threads x 8 elements each) ----
         {
             bf16_t va[8], vb[8];
@@ -88,7 +93,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(const GemmArgs g) {
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const int k = k0 + kk + e;
-                    va[e] = (m < g.M && k < g.K) ? g.A[(int64_t)m * g.lda + k] : (bf16_t)0;
+                    va[e] = (m < g.M && k < ke) ? g.A[(int64_t)m * g.lda + k] : (bf16_t)0;
                 }
 #pragma unroll
                 for (int e = 0; e < 8; ++e) sA[r][kk + e] = va[e];
@@ -98,7 +103,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(const GemmArgs g) {
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const int m = m0 + r + e;
-                    va[e] = (m < g.M && k < g.K) ? g.A[(int64_t)k * g.lda + m] : (bf16_t)0;
+                    va[e] = (m < g.M && k < ke) ? g.A[(int64_t)k * g.lda + m] : (bf16_t)0;
                 }
 #pragma unroll
                 for (int e = 0; e < 8; ++e) sA[r + e][kk] = va[e];
@@ -109,7 +114,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(const GemmArgs g) {
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const int k = k0 + kk + e;
-                    vb[e] = (n < g.N && k < g.K) ? g.B[(int64_t)n * g.ldb + k] : (bf16_t)0;
+                    vb[e] = (n < g.N && k < ke) ? g.B[(int64_t)n * g.ldb + k] : (bf16_t)0;
                 }
 #pragma unroll
                 for (int e = 0; e < 8; ++e) sB[r][kk + e] = vb[e];
@@ -119,7 +124,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(const GemmArgs g) {
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const int n = n0 + r + e;
-                    vb[e] = (n < g.N && k < g.K) ? g.B[(int64_t)k * g.ldb + n] : (bf16_t)0;
+                    vb[e] = (n < g.N && k < ke) ? g.B[(int64_t)k * g.ldb + n] : (bf16_t)0;
                 }
 #pragma unroll
                 for (int e = 0; e < 8; ++e) sB[r + e][kk] = vb[e];
@@ -149,7 +154,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(const GemmArgs g) {
         const int64_t o = (int64_t)i * g.ldc + j;
         const float x = acc[v];
         if (EPI == EPI_F32) {
-            g.C32[o] = x;
+            g.C32[o + (int64_t)blockIdx.z * g.zstride] = x;
         } else if (EPI == EPI_BIAS_F32) {
             g.C32[o] = x + g.bias[j];
         } else if (EPI == EPI_BIAS_GELU) {
@@ -160,6 +165,17 @@ __global__ __launch_bounds__(256) void gemm_bf16(const GemmArgs g) {
             g.C16[o] = f2bf(x * gelu_grad(g.Z[o]));
         }
     }
+}
+
+// C[i][j] = sum over z = 0..S-1, in that order, of the split-K partial tiles.
+__global__ void splitk_sum(int S, int M, int N, int64_t ldc, int64_t zstride, const float *part,
+                           float *C) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)M * N) return;
+    const int64_t o = (e / N) * ldc + e % N;
+    float v = part[o];
+    for (int z = 1; z < S; ++z) v += part[z * zstride + o];
+    C[o] = v;
 }
 
 // ---- element-wise training helpers ----

@@ -121,6 +121,7 @@ struct ohc_surrogate {
     std::vector<float *> Z;
     std::vector<bf16_t *> H;
     float *out = nullptr, *part = nullptr, *part2 = nullptr, *lsum = nullptr;
+    float *wsplit = nullptr;  // split-K partial tiles of the weight gradients
     int *d_fail = nullptr;
     unsigned int *d_nfail = nullptr;
     int64_t fail_cap = 0;
@@ -133,6 +134,8 @@ float *nm(ohc_surrogate *s, int which) {  // 0..3 in_*, 4..7 out_*
     const int64_t off[8] = {0, a, 2 * a, 3 * a, 4 * a, 4 * a + b, 4 * a + 2 * b, 4 * a + 3 * b};
     return s->d_norm + off[which];
 }
+
+constexpr int kMaxSplit = 16;  // split-K factor limit of the weight-gradient GEMMs
 
 int ensure_ws(ohc_surrogate *s, int64_t B) {
     if (B <= s->cap) return OHC_OK;
@@ -147,7 +150,8 @@ int ensure_ws(ohc_surrogate *s, int64_t B) {
                  bo = al(4 * B * s->nout), bdo = al(2 * B * s->nout),
                  bl = al(4 * (size_t)(nblk(B * s->nout) + 64)),
                  bp = bl + al(4 * (size_t)(B / ohcm::kRowChunk + 2) * std::max(W, s->nout));
-    const size_t total = bx + s->depth * (bz + bh) + bo + bdo + 2 * bh + bp + 256;
+    const size_t bw = al(4 * (size_t)kMaxSplit * W * std::max(std::max(W, s->nin), s->nout));
+    const size_t total = bx + s->depth * (bz + bh) + bo + bdo + 2 * bh + bp + 256 + bw;
     HIPCHK(c, hipMalloc(&s->d_ws, total));
     char *q = (char *)s->d_ws;
     s->X16 = (bf16_t *)q;
@@ -172,6 +176,8 @@ int ensure_ws(ohc_surrogate *s, int64_t B) {
     q += 256;
     s->part = (float *)q;
     s->part2 = (float *)(q + bl);
+    q += bp;
+    s->wsplit = (float *)q;
     s->cap = B;
     return OHC_OK;
 }
@@ -180,9 +186,32 @@ template <bool AK, bool BK, int EPI>
 hipError_t gemm(hipStream_t st, int M, int N, int K, const bf16_t *A, int64_t lda, const bf16_t *B,
                 int64_t ldb, float *C32, bf16_t *C16, int64_t ldc, const float *bias = nullptr,
                 const float *Zs = nullptr) {
-    ohcm::GemmArgs g{M, N, K, A, lda, B, ldb, C32, C16, ldc, bias, Zs};
+    ohcm::GemmArgs g{M, N, K, A, lda, B, ldb, C32, C16, ldc, bias, Zs, K, 0};
     const dim3 grid((unsigned)((N + ohcm::kTN - 1) / ohcm::kTN), (unsigned)((M + ohcm::kTM - 1) / ohcm::kTM));
     hipLaunchKernelGGL((ohcm::gemm_bf16<AK, BK, EPI>), grid, dim3(256), 0, st, g);
+    return hipGetLastError();
+}
+
+// Weight gradients dW = dZ^T H reduce over the batch (K = batch rows) into a
+// few 64 x 64 tiles; split K so that the launch fills the chip, each split
+// writing its own partial tile into ws, then add the partials in a fixed
+// order (deterministic).
+hipError_t gemm_wgrad(hipStream_t st, int M, int N, int K, const bf16_t *A, int64_t lda,
+                      const bf16_t *B, int64_t ldb, float *C32, int64_t ldc, float *ws) {
+    const int gx = (N + ohcm::kTN - 1) / ohcm::kTN, gy = (M + ohcm::kTM - 1) / ohcm::kTM;
+    int S = std::min(kMaxSplit, std::max(1, 256 / (gx * gy)));
+    S = std::min(S, std::max(1, K / 256));  // at least 256 batch rows per split
+    if (S <= 1) return gemm<false, false, ohcm::EPI_F32>(st, M, N, K, A, lda, B, ldb, C32, nullptr, ldc);
+    const int kc = ((K + S - 1) / S + ohcm::kTK - 1) / ohcm::kTK * ohcm::kTK;
+    S = (K + kc - 1) / kc;
+    const int64_t zs = (int64_t)M * ldc;
+    ohcm::GemmArgs g{M, N, K, A, lda, B, ldb, ws, nullptr, ldc, nullptr, nullptr, kc, zs};
+    hipLaunchKernelGGL((ohcm::gemm_bf16<false, false, ohcm::EPI_F32>), dim3(gx, gy, S), dim3(256), 0, st,
+                       g);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(ohcm::splitk_sum, dim3(nblk((int64_t)M * N)), dim3(256), 0, st, S, M, N, ldc,
+                       zs, ws, C32);
     return hipGetLastError();
 }
 
@@ -230,8 +259,8 @@ int forward_backward(ohc_surrogate *s, int64_t n, const float *X, const float *Y
     HIPCHK(c, hipGetLastError());
     // backward
     float *G = s->d_g;
-    HIPCHK(c, (gemm<false, false, ohcm::EPI_F32>(st, nout, W, (int)n, s->dOut, nout, s->H[L - 1], W,
-                                                 G + s->w_off[L], nullptr, W)));
+    HIPCHK(c, gemm_wgrad(st, nout, W, (int)n, s->dOut, nout, s->H[L - 1], W, G + s->w_off[L], W,
+                         s->wsplit));
     if ((rc = colsum(s, n, nout, s->dOut, G + s->b_off[L]))) return rc;
     bf16_t *dz = s->dZa, *dz2 = s->dZb;
     HIPCHK(c, (gemm<true, false, ohcm::EPI_GELU_BWD>(st, (int)n, W, nout, s->dOut, nout,
@@ -240,8 +269,7 @@ int forward_backward(ohc_surrogate *s, int64_t n, const float *X, const float *Y
     for (int l = L - 1; l >= 0; --l) {
         const bf16_t *Hin = l == 0 ? s->X16 : s->H[l - 1];
         const int K = l == 0 ? nin : W;
-        HIPCHK(c, (gemm<false, false, ohcm::EPI_F32>(st, W, K, (int)n, dz, W, Hin, K,
-                                                     G + s->w_off[l], nullptr, K)));
+        HIPCHK(c, gemm_wgrad(st, W, K, (int)n, dz, W, Hin, K, G + s->w_off[l], K, s->wsplit));
         if ((rc = colsum(s, n, W, dz, G + s->b_off[l]))) return rc;
         if (l > 0) {
             HIPCHK(c, (gemm<true, false, ohcm::EPI_GELU_BWD>(st, (int)n, W, W, dz, W,
