@@ -68,10 +68,6 @@ double to_double(const std::string &s, const char *what) {
         throw std::runtime_error(std::string("cannot read number '") + s + "' (" + what + ")");
     return v;
 }
-bool starts_kw(const std::string &u, const char *kw) {
-    size_t n = std::strlen(kw);
-    return u.size() >= n && u.compare(0, n, kw) == 0;
-}
 
 struct Section {
     std::vector<std::string> lines;
@@ -283,12 +279,36 @@ void change_unit(double order, bool molecules, int uE, bool third_body, double &
     }
 }
 
-std::vector<double> slash_values(const std::string &s, size_t pos) {
-    size_t a = s.find('/', pos), b = a == std::string::npos ? a : s.find('/', a + 1);
-    if (a == std::string::npos || b == std::string::npos)
-        throw std::runtime_error("missing /.../ values in '" + s + "'");
-    std::vector<double> out;
-    for (auto &t : split_ws(s.substr(a + 1, b - a - 1))) out.push_back(to_double(t, "aux"));
+// One group of a reaction's auxiliary line: a name followed by an optional
+// "/values/" field ("LOW / 1e14 0 0 /", "H2O/12/", or a bare "DUP").
+struct AuxGroup {
+    std::string name, vals;
+    bool has_vals = false;
+};
+std::vector<AuxGroup> aux_groups(const std::string &u) {
+    std::vector<AuxGroup> out;
+    const size_t n = u.size();
+    size_t i = 0;
+    auto ws = [&](size_t j) { return std::isspace((unsigned char)u[j]) != 0; };
+    while (i < n) {
+        while (i < n && ws(i)) ++i;
+        if (i >= n) break;
+        AuxGroup g;
+        const size_t a = i;
+        while (i < n && !ws(i) && u[i] != '/') ++i;
+        g.name = u.substr(a, i - a);
+        size_t j = i;
+        while (j < n && ws(j)) ++j;
+        if (j < n && u[j] == '/') {
+            const size_t b = u.find('/', j + 1);
+            if (b == std::string::npos) throw std::runtime_error("unterminated /.../ in '" + u + "'");
+            g.vals = u.substr(j + 1, b - j - 1);
+            g.has_vals = true;
+            i = b + 1;
+        }
+        if (g.name.empty()) throw std::runtime_error("/.../ values without a name in '" + u + "'");
+        out.push_back(g);
+    }
     return out;
 }
 
@@ -613,53 +633,46 @@ void parse_chemkin(const std::string &chem, const std::string *thermo,
         }
         if (pend.empty()) throw std::runtime_error("auxiliary line before any reaction: " + line);
         Pending &p = pend.back();
-        auto tok = split_ws(u);
-        if (starts_kw(u, "DUP")) continue;
-        if (starts_kw(u, "LOW")) {
-            p.low = slash_values(u, 0);
-            continue;
-        }
-        if (starts_kw(u, "HIGH")) {
-            p.high = slash_values(u, 0);
-            continue;
-        }
-        if (starts_kw(u, "TROE")) {
-            p.troe = slash_values(u, 0);
-            continue;
-        }
-        if (starts_kw(u, "SRI")) {
-            p.sri = slash_values(u, 0);
-            continue;
-        }
-        if (starts_kw(u, "REV")) {
-            p.revp = slash_values(u, 0);
-            continue;
-        }
-        if (starts_kw(u, "FORD") || starts_kw(u, "RORD")) {
-            auto v = slash_values(u, 0);
-            size_t a = u.find('/');
-            auto nm = split_ws(u.substr(a + 1, u.find('/', a + 1) - a - 1));
-            if (nm.size() != 2) throw std::runtime_error("bad FORD/RORD: " + line);
-            (starts_kw(u, "FORD") ? p.ford : p.rord)[nm[0]] = to_double(nm[1], "order");
-            continue;
-        }
-        if (starts_kw(u, "PLOG") || starts_kw(u, "CHEB") || starts_kw(u, "TCHEB") ||
-            starts_kw(u, "PCHEB") || starts_kw(u, "LT") || starts_kw(u, "RLT") ||
-            starts_kw(u, "JAN") || starts_kw(u, "FIT1") || starts_kw(u, "HV") ||
-            starts_kw(u, "TDEP") || starts_kw(u, "EXCI") || starts_kw(u, "MOME") ||
-            starts_kw(u, "XSMI") || starts_kw(u, "UNITS"))
-            throw std::runtime_error("unsupported CHEMKIN keyword in: " + line);
-        // third-body efficiencies "SP/val/ SP/val/"
-        std::string s = u;
-        size_t q = 0;
-        while (true) {
-            size_t a = s.find('/', q);
-            if (a == std::string::npos) break;
-            size_t b = s.find('/', a + 1);
-            if (b == std::string::npos) throw std::runtime_error("bad efficiency line: " + line);
-            std::string nm = trim(s.substr(q, a - q));
-            p.effs[nm] = to_double(s.substr(a + 1, b - a - 1), "efficiency");
-            q = b + 1;
+        // Every "NAME /values/" group of the line in turn, and bare keywords
+        // (DUP): a line may carry LOW together with TROE, or LOW followed by
+        // efficiencies (chemkinFileRead.cpp:256-263 handles each keyword found
+        // on the line).  Keywords match whole names only.
+        for (const AuxGroup &g : aux_groups(u)) {
+            const std::string &k = g.name;
+            auto vals = [&]() {
+                if (!g.has_vals) throw std::runtime_error("missing /.../ values after " + k + " in: " + line);
+                std::vector<double> v;
+                for (auto &t : split_ws(g.vals)) v.push_back(to_double(t, k.c_str()));
+                return v;
+            };
+            if (k == "DUP" || k == "DUPLICATE") {
+                if (g.has_vals) throw std::runtime_error("DUP takes no values: " + line);
+            } else if (k == "LOW") {
+                p.low = vals();
+            } else if (k == "HIGH") {
+                p.high = vals();
+            } else if (k == "TROE") {
+                p.troe = vals();
+            } else if (k == "SRI") {
+                p.sri = vals();
+            } else if (k == "REV") {
+                p.revp = vals();
+            } else if (k == "FORD" || k == "RORD") {
+                if (!g.has_vals) throw std::runtime_error("missing /.../ values after " + k + " in: " + line);
+                auto nm = split_ws(g.vals);
+                if (nm.size() != 2) throw std::runtime_error("bad FORD/RORD: " + line);
+                (k == "FORD" ? p.ford : p.rord)[nm[0]] = to_double(nm[1], "order");
+            } else if (k == "PLOG" || k == "CHEB" || k == "TCHEB" || k == "PCHEB" || k == "LT" ||
+                       k == "RLT" || k == "JAN" || k == "FIT1" || k == "HV" || k == "TDEP" ||
+                       k == "EXCI" || k == "MOME" || k == "XSMI" || k == "UNITS" ||
+                       k == "COLLEFF") {
+                throw std::runtime_error("unsupported CHEMKIN keyword " + k + " in: " + line);
+            } else if (g.has_vals) {
+                // third-body efficiency "SP/val/"
+                p.effs[k] = to_double(g.vals, "efficiency");
+            } else {
+                throw std::runtime_error("unrecognised auxiliary keyword " + k + " in: " + line);
+            }
         }
     }
     for (auto &p : pend) {
