@@ -77,11 +77,14 @@ inline int stage_buf(ohc_ctx *c, size_t slot, size_t bytes, void **out) {
 }
 
 // Bring one array to the device (host mode) or pass it through (device mode).
+// A block of rows (species fields with leading dimension ld) is copied row by
+// row, n elements each, so that the ld - n elements between two rows of the
+// caller's array are never written.
 struct Arr {
     const void *host = nullptr;
     void *dev = nullptr;
     size_t bytes = 0;
-    bool out = false;
+    size_t rows = 1, width = 0, pitch = 0;  // width/pitch in bytes when rows > 1
 };
 
 class Staging {
@@ -101,15 +104,37 @@ class Staging {
         int rc = stage_buf(c_, slot_++, bytes, &d);
         if (rc) return rc;
         if (in) HIPCHK(c_, hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, c_->stream));
-        if (out) arrs_.push_back(Arr{p, d, bytes, true});
+        if (out) arrs_.push_back(Arr{p, d, bytes, 1, bytes, bytes});
+        *dev = d;
+        return OHC_OK;
+    }
+    // rows x n elements of elem bytes with leading dimension ld (>= n)
+    int map_rows(const void *p, int64_t rows, int64_t n, int64_t ld, size_t elem, bool in,
+                 bool out, void **dev) {
+        const size_t bytes = elem * (size_t)(ld * (rows - 1) + n);
+        if (ld == n || rows <= 1 || !p || c_->opts.device_pointers)
+            return map(p, bytes, in, out, dev);
+        void *d = nullptr;
+        int rc = stage_buf(c_, slot_++, bytes, &d);
+        if (rc) return rc;
+        const size_t w = elem * (size_t)n, pitch = elem * (size_t)ld;
+        if (in)
+            HIPCHK(c_, hipMemcpy2DAsync(d, pitch, p, pitch, w, (size_t)rows,
+                                        hipMemcpyHostToDevice, c_->stream));
+        if (out) arrs_.push_back(Arr{p, d, bytes, (size_t)rows, w, pitch});
         *dev = d;
         return OHC_OK;
     }
     int finish() {
         if (c_->opts.device_pointers) return OHC_OK;
-        for (auto &a : arrs_)
-            HIPCHK(c_, hipMemcpyAsync(const_cast<void *>(a.host), a.dev, a.bytes,
-                                      hipMemcpyDeviceToHost, c_->stream));
+        for (auto &a : arrs_) {
+            if (a.rows > 1)
+                HIPCHK(c_, hipMemcpy2DAsync(const_cast<void *>(a.host), a.pitch, a.dev, a.pitch,
+                                            a.width, a.rows, hipMemcpyDeviceToHost, c_->stream));
+            else
+                HIPCHK(c_, hipMemcpyAsync(const_cast<void *>(a.host), a.dev, a.bytes,
+                                          hipMemcpyDeviceToHost, c_->stream));
+        }
         HIPCHK(c_, hipStreamSynchronize(c_->stream));
         return OHC_OK;
     }

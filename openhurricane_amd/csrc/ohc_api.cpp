@@ -260,18 +260,18 @@ int chem_integrate(ohc_ctx *c, int mode, int64_t n, const double *T, double *Tio
     if (!c->lane && !c->wave)
         return fail(c, OHC_ERR_UNSUPPORTED, "no device integrator for this mechanism");
     const size_t sv = sizeof(double) * (size_t)n;
-    const size_t sY = sizeof(double) * (size_t)(ldY * (ns - 1) + n);
     Staging st(c);
     void *dT, *dp, *drho, *dY, *ddt, *dsub, *dR = nullptr;
     if ((rc = st.map(mode == 0 ? (const void *)T : Tio, sv, true, mode == 1, &dT))) return rc;
     if ((rc = st.map(p, sv, true, false, &dp))) return rc;
     if ((rc = st.map(rho, sv, true, false, &drho))) return rc;
-    if ((rc = st.map(mode == 0 ? (const void *)Y : Yio, sY, true, mode == 1, &dY))) return rc;
+    if ((rc = st.map_rows(mode == 0 ? (const void *)Y : Yio, ns, n, ldY, sizeof(double), true,
+                          mode == 1, &dY)))
+        return rc;
     if ((rc = st.map(dt, sv, true, false, &ddt))) return rc;
     if ((rc = st.map(subDt, sv, true, true, &dsub))) return rc;
     if (mode == 0) {
-        const size_t sR = sizeof(double) * (size_t)(ldR * (ns - 1) + n);
-        if ((rc = st.map(R, sR, false, true, &dR))) return rc;
+        if ((rc = st.map_rows(R, ns, n, ldR, sizeof(double), false, true, &dR))) return rc;
     }
     ohcd::LaneParams P;
     std::memset(&P, 0, sizeof(P));
@@ -565,16 +565,21 @@ int ohc_ctx_autotune(ohc_ctx *c, int64_t n, const double *T, const double *p, co
     // private copies of the advanced state: the caller's arrays stay untouched
     std::vector<double> hT, hY, hsub;
     double *dT = nullptr, *dY = nullptr, *dsub = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    // every HIP failure below records the error and falls through to the
+    // common clean-up, which frees what was allocated and restores the grid
+    auto ok = [&](hipError_t e, const char *what) {
+        if (e == hipSuccess) return true;
+        if (!rc) rc = fail(c, OHC_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+        return false;
+    };
     if (dev) {
-        HIPCHK(c, hipMalloc((void **)&dT, sv));
-        HIPCHK(c, hipMalloc((void **)&dY, sY));
-        HIPCHK(c, hipMalloc((void **)&dsub, sv));
+        if (ok(hipMalloc((void **)&dT, sv), "hipMalloc") && ok(hipMalloc((void **)&dY, sY), "hipMalloc"))
+            ok(hipMalloc((void **)&dsub, sv), "hipMalloc");
     } else {
         hsub.assign((size_t)n, 0.0);
     }
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
+    if (!rc && ok(hipEventCreate(&e0), "hipEventCreate")) ok(hipEventCreate(&e1), "hipEventCreate");
     auto restore = [&]() -> int {
         if (dev) {
             HIPCHK(c, hipMemcpyAsync(dT, T, sv, hipMemcpyDeviceToDevice, c->stream));
@@ -598,17 +603,21 @@ int ohc_ctx_autotune(ohc_ctx *c, int64_t n, const double *T, const double *p, co
     // than the best so far (the step time is unimodal in the grid size)
     for (int w = c->max_per_cu; w >= 1 && !rc; --w) {
         c->grid = c->cus * w;
-        if (dev) HIPCHK(c, hipMemsetAsync(dsub, 0, sv, c->stream));
-        else std::fill(hsub.begin(), hsub.end(), 0.0);
+        if (dev) {
+            if (!ok(hipMemsetAsync(dsub, 0, sv, c->stream), "hipMemsetAsync")) break;
+        } else {
+            std::fill(hsub.begin(), hsub.end(), 0.0);
+        }
         // first flow step sets the sub-step memory and the cell order; the
         // second, from the same state, is timed
         if ((rc = restore()) || (rc = advance()) || (rc = restore())) break;
-        HIPCHK(c, hipEventRecord(e0, c->stream));
+        if (!ok(hipEventRecord(e0, c->stream), "hipEventRecord")) break;
         if ((rc = advance())) break;
-        HIPCHK(c, hipEventRecord(e1, c->stream));
-        HIPCHK(c, hipEventSynchronize(e1));
         float ms = 0.0f;
-        HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+        if (!ok(hipEventRecord(e1, c->stream), "hipEventRecord") ||
+            !ok(hipEventSynchronize(e1), "hipEventSynchronize") ||
+            !ok(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime"))
+            break;
         if (best == 0 || ms < best_t) {
             best = w;
             best_t = ms;
@@ -617,13 +626,11 @@ int ohc_ctx_autotune(ohc_ctx *c, int64_t n, const double *T, const double *p, co
         }
     }
     (void)hipStreamSynchronize(c->stream);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (dev) {
-        (void)hipFree(dT);
-        (void)hipFree(dY);
-        (void)hipFree(dsub);
-    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (dT) (void)hipFree(dT);
+    if (dY) (void)hipFree(dY);
+    if (dsub) (void)hipFree(dsub);
     if (rc || best == 0) {
         c->grid = keep;
         return rc ? rc : fail(c, OHC_ERR_DEVICE, "autotune found no working configuration");
@@ -679,9 +686,10 @@ int ohc_chem_source(ohc_ctx *c, int64_t n, const double *T, const double *p, con
     if ((rc = st.map(T, sv, true, false, &dT))) return rc;
     if ((rc = st.map(p, sv, true, false, &dp))) return rc;
     if ((rc = st.map(rho, sv, true, false, &drho))) return rc;
-    if ((rc = st.map(Y, sizeof(double) * (size_t)(ldY * (ns - 1) + n), true, false, &dY))) return rc;
-    if ((rc = st.map(R, sizeof(double) * (size_t)(ldR * (ns - 1) + n), false, true, &dR))) return rc;
-    if (diagJ && (rc = st.map(diagJ, sizeof(double) * (size_t)(ldD * (ns - 1) + n), false, true, &dD)))
+    if ((rc = st.map_rows(Y, ns, n, ldY, sizeof(double), true, false, &dY))) return rc;
+    if ((rc = st.map_rows(R, ns, n, ldR, sizeof(double), false, true, &dR))) return rc;
+    if (diagJ && (rc = st.map_rows(diagJ, ns - withoutLastSpec, n, ldD, sizeof(double), false,
+                                   true, &dD)))
         return rc;
     ohcd::SourceParams P{n,   (const double *)dT, (const double *)dp, (const double *)drho,
                          (const double *)dY, ldY, (double *)dR, ldR, (double *)dD, ldD,
@@ -703,10 +711,10 @@ static int eval_common(ohc_ctx *c, int64_t n, const double *y, int64_t ldy, cons
     if (!c->lane && n > 0x7fffffffll) return fail(c, OHC_ERR_ARG, "n too large");
     Staging st(c);
     void *dy, *dp, *df, *dJ = nullptr;
-    if ((rc = st.map(y, sizeof(double) * (size_t)(ldy * (N - 1) + n), true, false, &dy))) return rc;
+    if ((rc = st.map_rows(y, N, n, ldy, sizeof(double), true, false, &dy))) return rc;
     if ((rc = st.map(p, sizeof(double) * (size_t)n, true, false, &dp))) return rc;
-    if ((rc = st.map(f, sizeof(double) * (size_t)(ldf * (N - 1) + n), false, true, &df))) return rc;
-    if (J && (rc = st.map(J, sizeof(double) * (size_t)(ldJ * (N * N - 1) + n), false, true, &dJ)))
+    if ((rc = st.map_rows(f, N, n, ldf, sizeof(double), false, true, &df))) return rc;
+    if (J && (rc = st.map_rows(J, (int64_t)N * N, n, ldJ, sizeof(double), false, true, &dJ)))
         return rc;
     ohcd::EvalParams P{n,     (const double *)dy, ldy, (const double *)dp, (double *)df,
                        ldf,   (double *)dJ,       ldJ, c->opts.flow_type,  c->opts.jacobian,
@@ -741,7 +749,7 @@ int ohc_thermo_T_from_h(ohc_ctx *c, int64_t n, const double *h, const double *p,
     void *dh, *dY, *dT, *dI = nullptr;
     const size_t sv = sizeof(double) * (size_t)n;
     if ((rc = st.map(h, sv, true, false, &dh))) return rc;
-    if ((rc = st.map(Y, sizeof(double) * (size_t)(ldY * (ns - 1) + n), true, false, &dY))) return rc;
+    if ((rc = st.map_rows(Y, ns, n, ldY, sizeof(double), true, false, &dY))) return rc;
     if ((rc = st.map(T, sv, true, true, &dT))) return rc;
     if (iflag && (rc = st.map(iflag, sizeof(int32_t) * (size_t)n, false, true, &dI))) return rc;
     ohcd::ThParams P{n, (const double *)dh, (const double *)dY, ldY, (double *)dT, (int *)dI,
@@ -767,17 +775,17 @@ int ohc_mix_props(ohc_ctx *c, int64_t n, const double *T, const double *p, const
     if (!c->lane && !c->wave)
         return fail(c, OHC_ERR_UNSUPPORTED, "no device property kernels for this mechanism");
     Staging st(c);
-    const size_t sv = sizeof(double) * (size_t)n, sD = sizeof(double) * (size_t)(ldD * (ns - 1) + n);
+    const size_t sv = sizeof(double) * (size_t)n;
     void *dT, *dp, *dY, *dmu, *dk, *dD, *dcp, *dha, *dhk;
     if ((rc = st.map(T, sv, true, false, &dT))) return rc;
     if ((rc = st.map(p, sv, true, false, &dp))) return rc;
-    if ((rc = st.map(Y, sizeof(double) * (size_t)(ldY * (ns - 1) + n), true, false, &dY))) return rc;
+    if ((rc = st.map_rows(Y, ns, n, ldY, sizeof(double), true, false, &dY))) return rc;
     if ((rc = st.map(mu, sv, false, true, &dmu))) return rc;
     if ((rc = st.map(kappa, sv, false, true, &dk))) return rc;
-    if ((rc = st.map(Dk, sD, false, true, &dD))) return rc;
+    if ((rc = st.map_rows(Dk, ns, n, ldD, sizeof(double), false, true, &dD))) return rc;
     if ((rc = st.map(cp, sv, false, true, &dcp))) return rc;
     if ((rc = st.map(ha, sv, false, true, &dha))) return rc;
-    if ((rc = st.map(hk, sD, false, true, &dhk))) return rc;
+    if ((rc = st.map_rows(hk, ns, n, ldD, sizeof(double), false, true, &dhk))) return rc;
     ohcd::PropParams P{n,          (const double *)dT, (const double *)dp, (const double *)dY,
                        ldY,        (double *)dmu,      (double *)dk,       (double *)dD,
                        ldD,        (double *)dcp,      (double *)dha,      (double *)dhk,
