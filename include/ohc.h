@@ -36,6 +36,12 @@
  *   ohc_thermo_T_from_h
  *       thermoList::THa_p / THa_rho (src/speciesTransport/thermo/include/thermoList.inl:315-340,
  *       thermoList.cpp:504-533).
+ *   ohc_thermo_T_from_e
+ *       thermoList::TEa_rho / TEa_p (thermoList.inl:343-370 -> thermoList.cpp:504-533 with
+ *       ea_rho / ea_p and cv_rho / cv_p, thermoList.inl:145-170, :289-311); caller
+ *       rhoThermo::TEa (src/speciesTransport/basic/src/rhoThermo.cpp:87-93).
+ *   ohc_mix_thermo
+ *       thermoList::cp_p, cv_p, ha_p, ea_p (thermoList.cpp:165-290, thermoList.inl:43-311).
  *   ohc_mix_props
  *       mixture::gasProperties / transportList::muKappa / transportList::Df and
  *       CUDATransport::calcTransportPropertiesAsync
@@ -261,6 +267,19 @@ int ohc_eval_jacobian(ohc_ctx *ctx, int64_t n, const double *y, int64_t ldy,
  * [T_low, T_high]); iflag (nullable) receives 0/1/2 as thermoList::limit. */
 int ohc_thermo_T_from_h(ohc_ctx *ctx, int64_t n, const double *h, const double *p,
                         const double *Y, int64_t ldY, double *T_inout, int32_t *iflag);
+
+/* thermoList::TEa_p / TEa_rho: T from absolute internal energy e [J/kg]
+ * (e = ha - p/rho = ha - Rm T for the perfect gas), Newton on e(T) with
+ * de/dT = cv = cp - Rm, otherwise as ohc_thermo_T_from_h.  p_or_rho is not
+ * needed by the perfect-gas equation of state and may be NULL. */
+int ohc_thermo_T_from_e(ohc_ctx *ctx, int64_t n, const double *e, const double *p_or_rho,
+                        const double *Y, int64_t ldY, double *T_inout, int32_t *iflag);
+
+/* Mixture thermo at (T, p, Y): cp, cv [J/kg/K], ha, ea [J/kg]; any output may
+ * be NULL.  cv = cp - Rm, ea = ha - Rm T (perfect gas). */
+int ohc_mix_thermo(ohc_ctx *ctx, int64_t n, const double *T, const double *p,
+                   const double *Y, int64_t ldY, double *cp, double *cv, double *ha,
+                   double *ea);
 
 /* Mixture thermo + transport at (T, p, Y): any output may be NULL.
  * mu [kg/m/s] and kappa [W/m/K] by kinetic theory + Wilke; Dk [m^2/s]

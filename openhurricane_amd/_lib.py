@@ -93,6 +93,8 @@ _SIGS = [
     ("ohc_thermo_T_from_h", C.c_int, [_V, C.c_int64, _V, _V, _V, C.c_int64, _V, _V]),
     ("ohc_mix_props", C.c_int, [_V, C.c_int64, _V, _V, _V, C.c_int64, _V, _V, _V, C.c_int64,
                                 _V, _V, _V]),
+    ("ohc_thermo_T_from_e", C.c_int, [_V, C.c_int64, _V, _V, _V, C.c_int64, _V, _V]),
+    ("ohc_mix_thermo", C.c_int, [_V, C.c_int64, _V, _V, _V, C.c_int64, _V, _V, _V, _V]),
 ]
 
 _FP = C.POINTER(C.c_float)

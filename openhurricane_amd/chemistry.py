@@ -276,6 +276,24 @@ class ChemistrySource:
                                         self._p(flag, "int32") if want_flag else None), self._h)
         return (T, flag) if want_flag else T
 
+    # thermoList::TEa_p / TEa_rho
+    def T_from_e(self, e, p, Y, T0, want_flag=False):
+        n = e.shape[0]
+        T = T0.copy() if not self.device_pointers else T0.clone()
+        flag = self._zeros(n, np.int32) if want_flag else None
+        check(lib().ohc_thermo_T_from_e(self._h, n, self._p(e), self._p(p), self._p(Y),
+                                        _ld(Y, n), self._p(T),
+                                        self._p(flag, "int32") if want_flag else None), self._h)
+        return (T, flag) if want_flag else T
+
+    # thermoList::cp_p / cv_p / ha_p / ea_p
+    def mix_thermo(self, T, p, Y):
+        n = T.shape[0]
+        out = {k: self._zeros(n) for k in ("cp", "cv", "ha", "ea")}
+        check(lib().ohc_mix_thermo(self._h, n, self._p(T), self._p(p), self._p(Y), _ld(Y, n),
+                                   *[self._p(out[k]) for k in ("cp", "cv", "ha", "ea")]), self._h)
+        return out
+
     # transportList::muKappa + Df, thermoList cp / ha
     # chemistrySource::calculateSourceTerms(Imp): R [ns, n] and, with diag, the
     # (coupled) diagonal d(omega_k)/d(c_k) [ns or ns-1, n]

@@ -765,6 +765,7 @@ struct PropParams {
     int64_t ldD;
     double *cp, *ha, *hk;
     int prec;  // 1: species transport and mixture rules in FP32
+    double *cv, *ea;  // thermoList::cv_p, ea_p [J/kg/K, J/kg] (nullable)
 };
 
 // F = float evaluates the kinetic-theory species properties, Wilke's mixing
@@ -792,6 +793,21 @@ __global__ __launch_bounds__(256) void props_lane_kernel(const PropParams P) {
     });
     if (P.cp) P.cp[cell] = cpm;
     if (P.ha) P.ha[cell] = ham;
+    if (P.cv || P.ea) {
+        // perfect gas: cv = cp - Rm, ea = ha - p / rho = ha - Rm T
+        // (thermoList.inl:145-156, :289-298; speciesList.cpp:83-86, :106-116)
+        double Wm;
+        if constexpr (NS == 1) {
+            Wm = M::W[0];
+        } else {
+            double s = 0.0;
+            sfor<0, NS>([&](auto k) { s += yi[k] / M::W[k]; });
+            Wm = 1.0 / fmax(s, kVeryTiny);
+        }
+        const double Rm = kRu / Wm;
+        if (P.cv) P.cv[cell] = cpm - Rm;
+        if (P.ea) P.ea[cell] = ham - Rm * T;
+    }
     // Yi2Xi (speciesList.cpp:208-222)
     if constexpr (NS == 1) {
         xi[0] = 1.0;
@@ -926,12 +942,26 @@ __global__ __launch_bounds__(64) void source_lane_kernel(const SourceParams P) {
 // ---- thermoList::THa_p Newton inversion (thermoList.cpp:504-533) ----
 struct ThParams {
     int64_t n;
-    const double *h, *Y;
+    const double *h, *Y;  // h: target ha, or ea when energy = 1 [J/kg]
     int64_t ldY;
     double *T;
     int *iflag;
     double Tlow, Thigh;
+    int energy;  // 0: THa (F = ha, dF/dT = cp); 1: TEa (F = ea = ha - Rm T, dF/dT = cv = cp - Rm)
 };
+
+// Rm = Ru / W of a mass-fraction vector (speciesList::Rm / MWbyYi,
+// speciesList.cpp:83-86, :106-116), species in order.
+template <class M>
+__device__ __forceinline__ double mix_Rm(const double *yi) {
+    if constexpr (M::NS == 1) {
+        return kRu / M::W[0];
+    } else {
+        double s = 0.0;
+        sfor<0, M::NS>([&](auto k) { s += yi[k] / M::W[k]; });
+        return kRu / (1.0 / fmax(s, kVeryTiny));
+    }
+}
 
 template <class M>
 __global__ __launch_bounds__(256) void tfromh_lane_kernel(const ThParams P) {
@@ -942,6 +972,7 @@ __global__ __launch_bounds__(256) void tfromh_lane_kernel(const ThParams P) {
     sfor<0, NS>([&](auto k) { yi[k] = P.Y[k * P.ldY + cell]; });
     const double hstar = P.h[cell];
     const double T0 = P.T[cell], Ttol = T0 * 1.0e-4;
+    const double Rm = P.energy ? mix_Rm<M>(yi) : 0.0;
     double Test = T0, Tnew = T0;
     int iter = 0, flag = 1;
     do {
@@ -954,6 +985,10 @@ __global__ __launch_bounds__(256) void tfromh_lane_kernel(const ThParams P) {
             hm += th.H[k] / M::W[k] * yi[k];
             cpm += th.Cp[k] / M::W[k] * yi[k];
         });
+        if (P.energy) {
+            hm -= Rm * Test;
+            cpm -= Rm;
+        }
         double Tn = Test - (hm - hstar) / cpm;
         if (Tn < P.Tlow) {
             flag = 0;

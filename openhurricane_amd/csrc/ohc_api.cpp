@@ -735,9 +735,8 @@ int ohc_eval_jacobian(ohc_ctx *c, int64_t n, const double *y, int64_t ldy, const
     return eval_common(c, n, y, ldy, p, f, ldf, J, ldJ);
 }
 
-int ohc_thermo_T_from_h(ohc_ctx *c, int64_t n, const double *h, const double *p,
-                        const double *Y, int64_t ldY, double *T, int32_t *iflag) {
-    (void)p;
+static int t_from_f(ohc_ctx *c, int64_t n, const double *h, const double *Y, int64_t ldY,
+                    double *T, int32_t *iflag, int energy) {
     int rc = check_ctx(c);
     if (rc) return rc;
     const int ns = c->mech.ns;
@@ -753,15 +752,27 @@ int ohc_thermo_T_from_h(ohc_ctx *c, int64_t n, const double *h, const double *p,
     if ((rc = st.map(T, sv, true, true, &dT))) return rc;
     if (iflag && (rc = st.map(iflag, sizeof(int32_t) * (size_t)n, false, true, &dI))) return rc;
     ohcd::ThParams P{n, (const double *)dh, (const double *)dY, ldY, (double *)dT, (int *)dI,
-                     c->opts.T_low, c->opts.T_high};
+                     c->opts.T_low, c->opts.T_high, energy};
     if (c->lane) HIPCHK(c, c->lane->tfromh(P, c->stream));
     else HIPCHK(c, ohcd::wave_tfromh(P, c->wm, c->stream));
     return st.finish();
 }
 
-int ohc_mix_props(ohc_ctx *c, int64_t n, const double *T, const double *p, const double *Y,
-                  int64_t ldY, double *mu, double *kappa, double *Dk, int64_t ldD, double *cp,
-                  double *ha, double *hk) {
+int ohc_thermo_T_from_h(ohc_ctx *c, int64_t n, const double *h, const double *p,
+                        const double *Y, int64_t ldY, double *T, int32_t *iflag) {
+    (void)p;
+    return t_from_f(c, n, h, Y, ldY, T, iflag, 0);
+}
+
+int ohc_thermo_T_from_e(ohc_ctx *c, int64_t n, const double *e, const double *p_or_rho,
+                        const double *Y, int64_t ldY, double *T, int32_t *iflag) {
+    (void)p_or_rho;
+    return t_from_f(c, n, e, Y, ldY, T, iflag, 1);
+}
+
+static int mix_common(ohc_ctx *c, int64_t n, const double *T, const double *p, const double *Y,
+                      int64_t ldY, double *mu, double *kappa, double *Dk, int64_t ldD,
+                      double *cp, double *ha, double *hk, double *cv, double *ea) {
     int rc = check_ctx(c);
     if (rc) return rc;
     const int ns = c->mech.ns;
@@ -776,7 +787,7 @@ int ohc_mix_props(ohc_ctx *c, int64_t n, const double *T, const double *p, const
         return fail(c, OHC_ERR_UNSUPPORTED, "no device property kernels for this mechanism");
     Staging st(c);
     const size_t sv = sizeof(double) * (size_t)n;
-    void *dT, *dp, *dY, *dmu, *dk, *dD, *dcp, *dha, *dhk;
+    void *dT, *dp, *dY, *dmu, *dk, *dD, *dcp, *dha, *dhk, *dcv, *dea;
     if ((rc = st.map(T, sv, true, false, &dT))) return rc;
     if ((rc = st.map(p, sv, true, false, &dp))) return rc;
     if ((rc = st.map_rows(Y, ns, n, ldY, sizeof(double), true, false, &dY))) return rc;
@@ -786,13 +797,26 @@ int ohc_mix_props(ohc_ctx *c, int64_t n, const double *T, const double *p, const
     if ((rc = st.map(cp, sv, false, true, &dcp))) return rc;
     if ((rc = st.map(ha, sv, false, true, &dha))) return rc;
     if ((rc = st.map_rows(hk, ns, n, ldD, sizeof(double), false, true, &dhk))) return rc;
+    if ((rc = st.map(cv, sv, false, true, &dcv))) return rc;
+    if ((rc = st.map(ea, sv, false, true, &dea))) return rc;
     ohcd::PropParams P{n,          (const double *)dT, (const double *)dp, (const double *)dY,
                        ldY,        (double *)dmu,      (double *)dk,       (double *)dD,
                        ldD,        (double *)dcp,      (double *)dha,      (double *)dhk,
-                       c->opts.precision};
+                       c->opts.precision, (double *)dcv, (double *)dea};
     if (c->lane) HIPCHK(c, c->lane->props(P, c->stream));
     else HIPCHK(c, ohcd::wave_props(P, c->wm, c->stream));
     return st.finish();
+}
+
+int ohc_mix_thermo(ohc_ctx *c, int64_t n, const double *T, const double *p, const double *Y,
+                   int64_t ldY, double *cp, double *cv, double *ha, double *ea) {
+    return mix_common(c, n, T, p, Y, ldY, nullptr, nullptr, nullptr, n, cp, ha, nullptr, cv, ea);
+}
+
+int ohc_mix_props(ohc_ctx *c, int64_t n, const double *T, const double *p, const double *Y,
+                  int64_t ldY, double *mu, double *kappa, double *Dk, int64_t ldD, double *cp,
+                  double *ha, double *hk) {
+    return mix_common(c, n, T, p, Y, ldY, mu, kappa, Dk, ldD, cp, ha, hk, nullptr, nullptr);
 }
 
 }  // extern "C"

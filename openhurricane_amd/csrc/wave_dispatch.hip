@@ -97,6 +97,15 @@ __global__ __launch_bounds__(256) void props_wave_kernel(const PropParams P, con
         if (P.cp) P.cp[cell] = cpm;
         if (P.ha) P.ha[cell] = ham;
     }
+    if (P.cv || P.ea) {
+        // perfect gas: cv = cp - Rm, ea = ha - Rm T (thermoList.inl:145-156, :289-298)
+        const double Wm = ns == 1 ? S[SP_W * 64] : 1.0 / fmax(wsum(sp ? y / W : 0.0), kVeryTiny);
+        const double Rm = kRu / Wm;
+        if (lane == 0) {
+            if (P.cv) P.cv[cell] = cpm - Rm;
+            if (P.ea) P.ea[cell] = ham - Rm * T;
+        }
+    }
     if (!(P.mu || P.kappa || P.Dk)) return;
     // Yi2Xi
     double x;
@@ -170,6 +179,12 @@ __global__ __launch_bounds__(256) void tfromh_table_kernel(const ThParams P, con
     if (cell >= P.n) return;
     const double hstar = P.h[cell];
     const double T0 = P.T[cell], Ttol = T0 * 1.0e-4;
+    double Rm = 0.0;
+    if (P.energy) {  // TEa: F = ha - Rm T, dF/dT = cp - Rm (thermoList.inl:343-370)
+        double s = 0.0;
+        for (int k = 0; k < M.ns; ++k) s += P.Y[k * P.ldY + cell] / M.sp[SP_W * 64 + k];
+        Rm = kRu / (M.ns == 1 ? M.sp[SP_W * 64] : 1.0 / fmax(s, kVeryTiny));
+    }
     double Test = T0, Tnew = T0;
     int iter = 0, flag = 1;
     do {
@@ -182,6 +197,10 @@ __global__ __launch_bounds__(256) void tfromh_table_kernel(const ThParams P, con
             const double W = M.sp[SP_W * 64 + k], yk = P.Y[k * P.ldY + cell];
             hm += H / W * yk;
             cpm += Cp / W * yk;
+        }
+        if (P.energy) {
+            hm -= Rm * Test;
+            cpm -= Rm;
         }
         double Tn = Test - (hm - hstar) / cpm;
         if (Tn < P.Tlow) {

@@ -1764,14 +1764,32 @@ void orc_chem_source(const orc_mech *m, const ohc_opts *o, long n, const double 
 /* ------------------------------------------------------------------ */
 /* T from h: thermoList.cpp:504-533, limit :152-163                     */
 /* ------------------------------------------------------------------ */
-void orc_T_from_h(const orc_mech *m, const ohc_opts *o, long n, const double *h,
-                  const double *p, const double *Y, long ldY, double *T, int *iflag) {
+/* speciesList::MWbyYi (speciesList.cpp:106-116) -> Rm = Ru / W (speciesList.cpp:83-86) */
+static double mix_Rm_yi(const orc_mech *m, const double *yi) {
+    double Wm;
+    if (m->ns == 1) {
+        Wm = m->W[0];
+    } else {
+        double s = 0.0;
+        for (int k = 0; k < m->ns; ++k) s += yi[k] / m->W[k];
+        Wm = 1.0 / dmax(s, VERYTINY);
+    }
+    return ORC_RU / Wm;
+}
+
+/* thermoList::T (thermoList.cpp:504-533) with F = ha_p, dF/dT = cp_p (THa_p,
+ * thermoList.inl:328-340) or, energy = 1, F = ea_p = ha_p - Rm T and dF/dT =
+ * cv_p = cp_p - Rm (TEa_p / TEa_rho, thermoList.inl:343-370; perfect gas:
+ * p / rho = Rm T, cpMcvm = Rm, perfectGas.hpp:524-540); limit() clamps to
+ * [T_low, T_high] and sets iFlag 0 / 1 / 2 (thermoList.cpp:152-163). */
+static void T_from_f(const orc_mech *m, const ohc_opts *o, long n, const double *f_in,
+                     const double *Y, long ldY, double *T, int *iflag, int energy) {
     const int ns = m->ns;
-    (void)p;
     for (long cell = 0; cell < n; ++cell) {
         double yi[NMAX];
         for (int k = 0; k < ns; ++k) yi[k] = Y[k * ldY + cell];
-        const double f = h[cell];
+        const double f = f_in[cell];
+        const double Rm = energy ? mix_Rm_yi(m, yi) : 0.0;
         double T0 = T[cell], Test = T0, Tnew = T0, Ttol = T0 * 1.0e-4;
         int iter = 0, flag = 1;
         do {
@@ -1783,6 +1801,10 @@ void orc_T_from_h(const orc_mech *m, const ohc_opts *o, long n, const double *h,
             } else {
                 for (int k = 0; k < ns; ++k) hm += ha0(m, k, Test) * yi[k];
                 for (int k = 0; k < ns; ++k) cpm += cp0(m, k, Test) * yi[k];
+            }
+            if (energy) {
+                hm -= Rm * Test;
+                cpm -= Rm;
             }
             double Tn = Test - (hm - f) / cpm;
             if (Tn < o->T_low) {
@@ -1799,6 +1821,41 @@ void orc_T_from_h(const orc_mech *m, const ohc_opts *o, long n, const double *h,
         } while (fabs(Tnew - Test) > Ttol);
         T[cell] = Tnew;
         if (iflag) iflag[cell] = flag;
+    }
+}
+
+void orc_T_from_h(const orc_mech *m, const ohc_opts *o, long n, const double *h,
+                  const double *p, const double *Y, long ldY, double *T, int *iflag) {
+    (void)p;
+    T_from_f(m, o, n, h, Y, ldY, T, iflag, 0);
+}
+
+void orc_T_from_e(const orc_mech *m, const ohc_opts *o, long n, const double *e,
+                  const double *p_or_rho, const double *Y, long ldY, double *T, int *iflag) {
+    (void)p_or_rho;
+    T_from_f(m, o, n, e, Y, ldY, T, iflag, 1);
+}
+
+/* Mixture cp_p, cv_p, ha_p, ea_p (thermoList.cpp:165-290, thermoList.inl:145-156,
+ * :289-298): sums in species order, then cv = cp - Rm and ea = ha - Rm T. */
+void orc_mix_thermo(const orc_mech *m, long n, const double *T, const double *p,
+                    const double *Y, long ldY, double *cp, double *cv, double *ha, double *ea) {
+    const int ns = m->ns;
+    (void)p;
+    for (long cell = 0; cell < n; ++cell) {
+        double yi[NMAX];
+        for (int k = 0; k < ns; ++k) yi[k] = Y[k * ldY + cell];
+        const double Tc = T[cell];
+        double cpm = 0.0, hm = 0.0;
+        for (int k = 0; k < ns; ++k) {
+            cpm += cp0(m, k, Tc) * yi[k];
+            hm += ha0(m, k, Tc) * yi[k];
+        }
+        const double Rm = mix_Rm_yi(m, yi);
+        if (cp) cp[cell] = cpm;
+        if (cv) cv[cell] = cpm - Rm;
+        if (ha) ha[cell] = hm;
+        if (ea) ea[cell] = hm - Rm * Tc;
     }
 }
 

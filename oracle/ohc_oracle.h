@@ -16,7 +16,8 @@
  *   integrator    scipy Radau at rtol 1e-12 (accuracy), conservation; the
  *                 trapezoidal rule at second order; BDF23 between step
  *                 tolerances 1e-3 and 1e-4 (and its underflow at 1e-8)
- *   T(h)          round trip h(T(h)) = h and limiter flags
+ *   T(h), T(e)    round trips h(T(h)) = h, e(T(e)) = e, and limiter flags
+ *   cv, ea        cv = de/dT by finite differences, ea = ha - p/rho (perfect gas)
  *   transport     single-species and binary closed forms, symmetry
  */
 #ifndef OHC_ORACLE_H
@@ -76,6 +77,14 @@ void orc_chem_source(const orc_mech *m, const ohc_opts *o, long n, const double 
 /* thermoList::THa_p over n cells. */
 void orc_T_from_h(const orc_mech *m, const ohc_opts *o, long n, const double *h,
                   const double *p, const double *Y, long ldY, double *T, int *iflag);
+
+/* thermoList::TEa_p / TEa_rho over n cells (internal energy e [J/kg]). */
+void orc_T_from_e(const orc_mech *m, const ohc_opts *o, long n, const double *e,
+                  const double *p_or_rho, const double *Y, long ldY, double *T, int *iflag);
+
+/* Mixture cp_p, cv_p [J/kg/K], ha_p, ea_p [J/kg] (any NULL). */
+void orc_mix_thermo(const orc_mech *m, long n, const double *T, const double *p,
+                    const double *Y, long ldY, double *cp, double *cv, double *ha, double *ea);
 
 /* Mixture cp, ha, species ha, mu, kappa and mixture-averaged D_k (any NULL). */
 void orc_mix_props(const orc_mech *m, long n, const double *T, const double *p,

@@ -51,6 +51,8 @@ def lib():
         L.orc_chem_source.argtypes = [_V, _V, C.c_long, _V, _V, _V, _V, C.c_long, _V, C.c_long,
                                       _V, C.c_long, C.c_int]
         L.orc_T_from_h.argtypes = [_V, _V, C.c_long, _V, _V, _V, C.c_long, _V, _V]
+        L.orc_T_from_e.argtypes = [_V, _V, C.c_long, _V, _V, _V, C.c_long, _V, _V]
+        L.orc_mix_thermo.argtypes = [_V, C.c_long, _V, _V, _V, C.c_long, _V, _V, _V, _V]
         L.orc_mix_props.argtypes = [_V, C.c_long, _V, _V, _V, C.c_long, _V, _V, _V, C.c_long,
                                     _V, _V, _V]
         _L = L
@@ -165,6 +167,20 @@ class Oracle:
         fl = np.zeros(n, dtype=np.int32)
         lib().orc_T_from_h(self._h, C.byref(opts), n, _p(h), _p(p), _p(Y), n, _p(T), _p(fl))
         return T, fl
+
+    def T_from_e(self, opts, e, p, Y, T0):
+        n = e.shape[0]
+        T = T0.copy()
+        fl = np.zeros(n, dtype=np.int32)
+        lib().orc_T_from_e(self._h, C.byref(opts), n, _p(e), _p(p), _p(Y), n, _p(T), _p(fl))
+        return T, fl
+
+    def mix_thermo(self, T, p, Y):
+        n = T.shape[0]
+        out = {k: np.zeros(n) for k in ["cp", "cv", "ha", "ea"]}
+        lib().orc_mix_thermo(self._h, n, _p(T), _p(p), _p(Y), n, *[_p(out[k]) for k in
+                                                                  ["cp", "cv", "ha", "ea"]])
+        return out
 
     def mix_props(self, T, p, Y, hk=False):
         n = T.shape[0]

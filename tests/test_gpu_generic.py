@@ -349,3 +349,44 @@ def test_bdf23_options_are_checked(h2mech):
         ChemistrySource(h2mech, opts=_lib.default_opts(integrator=3, precision=1))
     cs = ChemistrySource(h2mech, opts=_lib.default_opts(integrator=3))
     assert cs.kernel_name.startswith("wave:"), cs.kernel_name
+
+
+@pytest.mark.parametrize("which", ["h2air_lane", "h2air", "syn32", "syn54"])
+def test_energy_forms_match_oracle(which, h2mech, h2oracle, syn32, syn32_oracle):
+    """thermoList::cv_p / ea_p and TEa (T from internal energy) on the
+    compiled H2-air kernels and on the run-time-table kernels, against the
+    oracle: cv, ea to 1e-12; T(e) to 1e-12 with identical limiter flags."""
+    from oracle import Oracle
+
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource, Mechanism
+    from openhurricane_amd.sampler import h2air_states, synthetic_workload
+
+    if which.startswith("h2air"):
+        mech, orc = h2mech, h2oracle
+    elif which == "syn32":
+        mech, orc = syn32, syn32_oracle
+    else:
+        mech = Mechanism.synthetic(54, 270, seed=2)
+        orc = Oracle(mech)
+    t = mech.tables()
+    o = _opts() if which != "h2air_lane" else _lib.default_opts()
+    cs = ChemistrySource(mech, opts=o)
+    assert cs.kernel_name.startswith("lane:" if which == "h2air_lane" else "wave:"), cs.kernel_name
+    n = 512
+    if which.startswith("h2air"):
+        st = h2air_states(mech.species, t["W"], n, seed=29, T_range=(300.0, 3500.0))
+    else:
+        st = synthetic_workload(mech.species, t["W"], n, seed=29)
+    Y = np.ascontiguousarray(st["Y"])
+    out = cs.mix_thermo(st["T"], st["p"], Y)
+    ref = orc.mix_thermo(st["T"], st["p"], Y)
+    for k in ("cp", "cv", "ha", "ea"):
+        np.testing.assert_allclose(out[k], ref[k], rtol=1e-12, atol=1e-9 * np.abs(ref[k]).max(),
+                                   err_msg=k)
+    T0 = st["T"] + 300.0
+    Tg, fl = cs.T_from_e(ref["ea"], st["p"], Y, T0, want_flag=True)
+    To, flo = orc.T_from_e(o, ref["ea"], st["p"], Y, T0)
+    np.testing.assert_allclose(Tg, To, rtol=1e-12)
+    np.testing.assert_array_equal(fl, flo)
+    np.testing.assert_allclose(Tg, st["T"], rtol=1e-4)
