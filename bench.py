@@ -1,16 +1,25 @@
-"""Benchmark: chem cells/s per flow step on one GPU (plain FP64 integrator).
+"""Benchmark: chem cells/s per flow step on one GPU (plain FP64 integrator),
+and the gated surrogate against it.
 
-Workload (the 1,048,576-cell H2-air configuration): 9 species / 19 reactions,
-cells seeded with T ~ U(900, 2500) K, p ~ U(0.5, 5) atm, H2/air mixture
-fraction Z ~ U(0, 1) plus 1e-6 radical seeds.  One step = one flow step of
-the chemistry path on device-resident SoA fields: restore (T, Y) from
-pristine copies, chemistrySource::solveUpdate over dt = 1e-6 s with the
-ImplicitEuler integrator (rtol 1e-6, atol 1e-10), then mixture viscosity,
-conductivity and mixture-averaged D_k at the new state.  The per-cell
-sub-step memory (dtInit) and the cost-ordered cell schedule carry over from
-step to step, as they do between flow steps.
+Headline workload (default): the synthetic 32-species / 150-reaction
+mechanism, 4,194,304 cells -- 20 % ignition-front states (T 1100-1600 K fresh
+mixtures), 40 % cold (300-1000 K) and 40 % burnt (pre-integrated 1e-2 s from
+1800-2600 K) -- advanced over dt = 1e-6 s by chemistrySource::solveUpdate with
+the ImplicitEuler integrator (Newton rtol 1e-6 / atol 1e-10, step error at the
+reference defaults RTolIncrement 1e-3 / ATolIncrement 1e-7) and the reference's
+Jacobian (its dKp/dT sign convention), one persistent work-queue kernel with
+the autotuned grid.  One step = one flow step on device-resident SoA fields:
+restore (T, Y) from pristine copies and advance; the per-cell sub-step memory
+(dtInit) and the cost-ordered cell schedule carry over from step to step.
 
-Prints ONE JSON line (see the repository's measurement notes for the fields).
+The same JSON line carries, measured in the same run: the same workload with
+each step's output fed to the next (consecutive flow steps) and the first,
+schedule-less call; the 1,048,576-cell H2-air workload (with mu, kappa, D_k,
+the FP32 split and the reference's other integrators); the gated surrogate on
+16,777,216 training pairs and 4,194,304 held-out cells; and the FP64 CPU
+restatement on all host cores over a stratified 65,536-cell sample.
+
+Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -27,203 +36,98 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "chem cells/s per flow step on 1 GPU; surrogate vs plain at ≤1e-3 rel err"
-FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix) peak
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
+SYNTH = {"syn32": (32, 150, 1, 1 << 22, 0.2), "syn54": (54, 270, 2, 1 << 23, 0.3)}
+# step-error tolerances of the synthetic workloads: the reference defaults
+SYN_TOL = dict(atol=1e-10, rtol=1e-6, atol_inc=1e-7, rtol_inc=1e-3)
+H2_TOL = dict(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6)
 
 
-def measured_traffic():
-    """HBM bytes per integrator launch from the committed rocprofv3 PMC summary."""
-    f = ROOT / "profiles" / "ie_lane_h2air_1M_pmc.txt"
+def log(*a):
+    print("bench:", *a, file=sys.stderr, flush=True)
+
+
+def profile_figure(name, key):
+    """A figure from a committed rocprofv3 summary under profiles/ (HBM bytes
+    per launch from the FETCH_SIZE / WRITE_SIZE passes), or None."""
     try:
-        for line in f.read_text().splitlines():
-            if line.startswith("traffic_bytes_per_launch"):
+        for line in (ROOT / "profiles" / name).read_text().splitlines():
+            if line.startswith(key):
                 return float(line.split()[1])
     except OSError:
         pass
     return None
 
 
-def cpu_baseline(mech, opts, states, n_sample, threads):
+def cpu_info():
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "available": len(os.sched_getaffinity(0)), "model": model}
+
+
+def stratified(kind, m):
+    """Indices of m cells with the workload's own share of each cell kind
+    (front / cold / burnt), interleaved so that contiguous thread ranges get
+    the same mix."""
+    kinds = np.unique(kind)
+    picks = []
+    for k in kinds:
+        idx = np.nonzero(kind == k)[0]
+        take = int(round(m * len(idx) / len(kind)))
+        picks.append(idx[:take])
+    sel = np.concatenate(picks)
+    rng = np.random.default_rng(5)
+    return np.sort(sel[rng.permutation(len(sel))][:m])
+
+
+def cpu_baseline(mech, opts, st, idx, threads, label):
+    """The FP64 CPU restatement of the reference algorithm (the oracle,
+    compiled -O3) on `threads` host threads, one contiguous cell range per
+    thread as the reference runs one MPI rank per core; the first call sets
+    the sub-step memory, the second (a flow step like the GPU's timed ones)
+    is timed."""
     sys.path.insert(0, str(ROOT / "oracle"))
     from oracle import Oracle
 
     orc = Oracle(mech)
-    sl = slice(0, n_sample)
-    T, p, rho = (np.ascontiguousarray(states[k][sl]) for k in ("T", "p", "rho"))
-    Y = np.ascontiguousarray(states["Y"][:, sl])
-    dt = np.full(n_sample, 1e-6)
-    # first call establishes the per-cell sub-step memory, the second is timed
+    T, p, rho = (np.ascontiguousarray(st[k][idx]) for k in ("T", "p", "rho"))
+    Y = np.ascontiguousarray(st["Y"][:, idx])
+    m = len(idx)
+    dt = np.full(m, 1e-6)
     _, _, sub, _ = orc.solve_update(opts, T, p, rho, Y, dt, 1.0, nthreads=threads)
     t0 = time.perf_counter()
     orc.solve_update(opts, T, p, rho, Y, dt, 1.0, subDt=sub, nthreads=threads)
     el = time.perf_counter() - t0
-    return {"value": n_sample / el, "unit": "cells/s", "cores": threads, "kind": "port",
-            "sample": f"first {n_sample} cells of the same seeded workload (second flow step, "
-                      f"carried sub-step memory), FP64 oracle, {threads} host threads, "
-                      f"{el:.2f} s wall"}
+    log(f"cpu {label}: {m} cells on {threads} threads, {el:.2f} s")
+    return m / el, el
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cells", type=int, default=None,
-                    help="cells (default: 1,048,576 for h2air, the workload's own size otherwise)")
-    ap.add_argument("--cpu-sample", type=int, default=32768,
-                    help="cells of the CPU-baseline sample (about 10-30 s of host work)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--schedule", type=int, default=1,
-                    help="1: longest-first cell order from the previous step's per-cell cost")
-    ap.add_argument("--reset-subdt", action="store_true",
-                    help="start every step from subDt = dt (first-call semantics)")
-    ap.add_argument("--workload", default="h2air", choices=["h2air", "syn32", "syn54"],
-                    help="h2air: 9-species H2-air (default); syn32 / syn54: synthetic "
-                         "32-species/150-reaction or 54/270 mechanisms on the generic kernel")
-    ap.add_argument("--no-surrogate", action="store_true",
-                    help="skip the compact gated-surrogate leg (about a minute)")
-    ap.add_argument("--no-split", action="store_true",
-                    help="skip the FP32 split-precision comparison")
-    ap.add_argument("--jacobian", type=int, default=0,
-                    help="0: reference Jacobian (its dKp/dT sign convention); 1: exact")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
-              "the chemistry path shards by contiguous cell ranges with no communication, "
-              "see the design notes", file=sys.stderr)
-        sys.exit(2)
-
-    if args.workload != "h2air":
-        return bench_synthetic(args)
-
-    import torch
-
-    from openhurricane_amd import _lib
-    from openhurricane_amd.chemistry import ChemistrySource, Mechanism
-    from openhurricane_amd.flops import integrator_flops, work_flops
-    from openhurricane_amd.sampler import h2air_states
-
-    dev = torch.device("cuda:0")
-    mech = Mechanism.h2air()
-    tab = mech.tables()
-    n = args.cells or (1 << 20)
-    st = h2air_states(mech.species, tab["W"], n, seed=2024)
-    opts = _lib.default_opts(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6,
-                             device_pointers=1, schedule=args.schedule, jacobian=args.jacobian)
-    cs = ChemistrySource(mech, device=0, opts=opts)
-    stream = torch.cuda.current_stream(dev)
-    cs.set_stream(stream.cuda_stream)
-    g = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in st.items()}
-    T0, Y0 = g["T"].clone(), g["Y"].clone()
-    T, Y = g["T"].clone(), g["Y"].clone()
-    p, rho = g["p"], g["rho"]
-    dt = torch.full((n,), 1e-6, dtype=torch.float64, device=dev)
-    sub0 = torch.zeros(n, dtype=torch.float64, device=dev)
-    sub = sub0.clone()
-    tune = autotune(cs, g, dt)
-
-    def step(ev=None):
-        # every step starts from the same (T, Y); the per-cell sub-step memory
-        # (chemistrySource dtInit) carries over between flow steps as in the reference
-        T.copy_(T0)
-        Y.copy_(Y0)
-        if args.reset_subdt:
-            sub.copy_(sub0)
-        if ev:
-            ev[0].record(stream)
-        cs.solve_update(T, p, rho, Y, dt, 1.0, subDt=sub, stats=False)
-        if ev:
-            ev[1].record(stream)
-        props = cs.mix_props(T, p, Y, mu=True, kappa=True, Dk=True, cp=False, ha=False)
-        return props
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(evs[i])
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    ms = el * 1e3 / args.steps
-    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
-
-    # counters of one more advance from the same state (same work as a timed step)
-    T.copy_(T0)
-    Y.copy_(Y0)
-    if args.reset_subdt:
-        sub.copy_(sub0)
-    sub_keep = sub.clone()
-    _, stats = cs.solve_update(T, p, rho, Y, dt, 1.0, subDt=sub, stats=True)
-    sub.copy_(sub_keep)
-    fl = work_flops(tab, stats)
-    achieved = fl / (kern_ms * 1e-3) / 1e12
-    roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": measured_traffic(),
-            "kernel": "ie_lane_kernel<Mech_h2air>", "kernel_ms": kern_ms, "autotune": tune,
-            "flops_per_launch": fl, "newton_iters": stats["newton_iters"],
-            "steps": stats["steps"], "flops_per_newton": integrator_flops(tab)["F_newton"]}
-    split = None if args.no_split else split_precision(args, mech, g, T0, Y0, dt, stream)
-    others = None if args.no_split else other_integrators(mech, g, T0, Y0, dt, stream)
-    surr = None if args.no_surrogate else surrogate_leg()
-    cpu = None
-    if not args.no_cpu:
-        threads = min(16, len(os.sched_getaffinity(0)))
-        cpu = cpu_baseline(mech, _lib.default_opts(atol=1e-10, rtol=1e-6, atol_inc=1e-10,
-                                                   rtol_inc=1e-6),
-                           st, args.cpu_sample, threads)
-    line = {
-        "metric": METRIC, "value": n / (ms * 1e-3), "unit": "cells/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "H2-air 9sp/19rxn, 1,048,576 cells, dt 1e-6 s, plain FP64 "
-                               "ImplicitEuler (rtol 1e-6, atol 1e-10) + mu/kappa/D_k",
-                   "cells": n, "mechanism": "h2air", "integrator": "ImplicitEuler",
-                   "parallelism": "single GPU"},
-        "roofline": roof, "cpu_baseline": cpu, "split_precision": split,
-        "other_integrators": others, "surrogate": surr,
-    }
-    print(json.dumps(line), flush=True)
-
-
-SURR_PAIRS, SURR_EVAL, SURR_EPOCHS = 1 << 22, 1 << 20, 10
-
-
-def surrogate_leg():
-    """Gated surrogate + exact fallback against the plain FP64 integrator, at a
-    compact scale of the 32-species surrogate workload (the full-scale run is
-    `python -m openhurricane_amd.surrogate_workflow`): two 34->256^3->33 BF16
-    networks trained on the GPU on pairs the GPU integrator generates, then
-    timed and checked on held-out cells of the same distribution.  A failure
-    here is reported in the line, never raised."""
-    try:
-        from openhurricane_amd.surrogate_workflow import run
-
-        r = run(["--pairs", str(SURR_PAIRS), "--eval", str(SURR_EVAL), "--epochs", str(SURR_EPOCHS),
-                 "--peer-tol", "1e-5", "1e-4"])
-    except Exception as e:  # the plain-path line must still be printed
-        return {"error": f"{type(e).__name__}: {e}"}
-    gated = []
-    for c in r["gated"]:
-        gated.append({"gate": ("element change <= %g" % c["gate_tol"] if c["gate_tol"] < 1e29
-                               else "none") + ("" if c.get("peer_tol") is None else
-                                               ", peer agreement %g" % c["peer_tol"])
-                      + (", FP8" if c.get("precision") else ""),
-                      "cells_per_s": c["cells_per_s"], "speedup_vs_plain": c["speedup"],
-                      "fallback_frac": c["fallback_frac"], "max_rel_err": max(
-                          c["max_rel_err_T"], c["max_rel_err_Ymajor"]),
-                      "max_rel_err_T": c["max_rel_err_T"]})
-    return {"workload": f"synthetic 32sp/150rxn, {SURR_PAIRS:,} training pairs from the GPU "
-                        f"integrator, {SURR_EVAL:,} held-out cells (20% ignition front), dt 1e-6 s; "
-                        f"two 34->256^3->33 GELU networks, BF16 on MFMA, {SURR_EPOCHS} epochs",
-            "plain_cells_per_s": r["plain"]["cells_per_s"],
-            "network_alone_cells_per_s": r["inference_bf16"]["cells_per_s"],
-            "seconds_train": r["seconds_train"], "gated": gated,
-            "target": ">= 5x the plain path at <= 1e-3 max relative error (T, Y >= 1e-3)"}
+def cpu_legs(mech, opts, st, m, m_single):
+    info = cpu_info()
+    cores = info["available"]
+    kind = st.get("kind")
+    if kind is None:
+        idx = np.arange(m)
+        idx1 = np.arange(m_single)
+        how = f"first {m} cells"
+    else:
+        idx = stratified(kind, m)
+        idx1 = idx[:: max(1, len(idx) // m_single)][:m_single]
+        how = f"stratified {m} cells (same front / cold / burnt shares)"
+    v, el = cpu_baseline(mech, opts, st, idx, cores, "all cores")
+    v1, el1 = cpu_baseline(mech, opts, st, idx1, 1, "one core")
+    return {"value": v, "unit": "cells/s", "cores": cores, "kind": "port",
+            "sample": f"{how} of the same seeded workload, second flow step (carried sub-step "
+                      f"memory), FP64 oracle -O3 on {cores} host threads, {el:.2f} s wall",
+            "single_core_cells_per_s": v1,
+            "single_core_sample": f"{len(idx1)} cells of the same sample, {el1:.2f} s",
+            "nproc": info["nproc"], "cpu_model": info["model"]}
 
 
 def autotune(cs, g, dt, m=1 << 20):
@@ -234,6 +138,284 @@ def autotune(cs, g, dt, m=1 << 20):
     sl = {k: (v[:, :m] if v.dim() == 2 else v[:m]).contiguous() for k, v in g.items()}
     w, ms = cs.autotune(sl["T"], sl["p"], sl["rho"], sl["Y"], dt[:m].contiguous(), 1.0)
     return {"waves_per_cu": w, "sample_cells": m, "sample_step_ms": ms}
+
+
+def timed_steps(cs, g, T0, Y0, dt, sub, steps, warmup, stream, after=None):
+    """W warm-up and K timed flow steps from (T0, Y0) with the carried
+    sub-step memory; returns (ms per step by wall clock, mean integrator
+    kernel ms by HIP events on the context stream)."""
+    import torch
+
+    T, Y = T0.clone(), Y0.clone()
+
+    def step(ev=None):
+        T.copy_(T0)
+        Y.copy_(Y0)
+        if ev:
+            ev[0].record(stream)
+        cs.solve_update(T, g["p"], g["rho"], Y, dt, 1.0, subDt=sub, stats=False)
+        if ev:
+            ev[1].record(stream)
+        if after:
+            after(T, Y)
+
+    for i in range(warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"warm-up step {i} done")
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step(evs[i])
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    log(f"{steps} timed steps: {el:.2f} s")
+    return el * 1e3 / steps, float(np.mean([a.elapsed_time(b) for a, b in evs]))
+
+
+def counters(cs, g, T0, Y0, dt, sub):
+    """Integrator counters of one more advance from the same state and
+    sub-step memory (the work of a timed step)."""
+    T, Y = T0.clone(), Y0.clone()
+    keep = sub.clone()
+    _, stats = cs.solve_update(T, g["p"], g["rho"], Y, dt, 1.0, subDt=sub, stats=True)
+    sub.copy_(keep)
+    return stats
+
+
+def roofline(tab, stats, kern_ms, kernel, profile):
+    from openhurricane_amd.flops import integrator_flops, work_flops
+
+    fl = work_flops(tab, stats)
+    achieved = fl / (kern_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": profile_figure(profile, "traffic_bytes_per_launch"),
+            "traffic_source": f"profiles/{profile} (rocprofv3 FETCH_SIZE + WRITE_SIZE passes)",
+            "kernel": kernel, "kernel_ms": kern_ms, "flops_per_launch": fl,
+            "newton_iters": stats["newton_iters"], "steps": stats["steps"],
+            "rhs_evals": stats["rhs_evals"],
+            "flops_per_newton": integrator_flops(tab)["F_newton"],
+            "note": "FP64 vector (VALU) bound: algorithmic flops of the reference algorithm "
+                    "(counted Newton iterations x per-iteration model) / integrator kernel time"}
+
+
+def consecutive(cs, g, T0, Y0, dt, steps, stream):
+    """Flow steps on which each step's output state is the next step's input
+    (no reset), with fresh sub-step memory and schedule at the start: the
+    first call (index order, no sub-step memory) and the mean of the rest."""
+    import torch
+
+    n = T0.shape[0]
+    T, Y = T0.clone(), Y0.clone()
+    sub = torch.zeros(n, dtype=torch.float64, device=T0.device)
+    times = []
+    for _ in range(steps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cs.solve_update(T, g["p"], g["rho"], Y, dt, 1.0, subDt=sub, stats=False)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    log(f"consecutive steps: {[round(t, 3) for t in times]}")
+    return {"first_call_cells_per_s": n / times[0],
+            "value": n / float(np.mean(times[1:])), "unit": "cells/s", "steps": steps,
+            "note": "flow steps chained (each step's output is the next step's input), "
+                    "sub-step memory and schedule carried; first_call = the first step of "
+                    "the chain, index order and no sub-step memory"}
+
+
+def build_synthetic(name, n):
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource, Mechanism
+    from openhurricane_amd.sampler import synthetic_workload
+
+    ns, nr, seed, default_cells, front = SYNTH[name]
+    mech = Mechanism.synthetic(ns, nr, seed=seed)
+    tab = mech.tables()
+    # the burnt-state pool is input preparation: exact Jacobian (fast Newton)
+    burner = ChemistrySource(mech, opts=_lib.default_opts(**dict(SYN_TOL, jacobian=1)))
+
+    def burn(T, p, rho, Y, dt):
+        burner.solve_update(T, p, rho, Y, dt, 1.0)
+
+    log(f"building {n} cells ({name})")
+    st = synthetic_workload(mech.species, tab["W"], n, seed=2024, front=front, burn=burn)
+    return mech, tab, st, front
+
+
+def bench_synthetic(args):
+    import torch
+
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource
+
+    ns, nr, seed, default_cells, _ = SYNTH[args.workload]
+    n = args.cells or default_cells
+    mech, tab, st, front = build_synthetic(args.workload, n)
+    dev = torch.device("cuda:0")
+    opts = _lib.default_opts(**SYN_TOL, jacobian=args.jacobian, device_pointers=1,
+                             schedule=args.schedule)
+    cs = ChemistrySource(mech, device=0, opts=opts)
+    stream = torch.cuda.current_stream(dev)
+    cs.set_stream(stream.cuda_stream)
+    g = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in st.items()
+         if k != "kind"}
+    T0, Y0 = g["T"].clone(), g["Y"].clone()
+    dt = torch.full((n,), 1e-6, dtype=torch.float64, device=dev)
+    sub = torch.zeros(n, dtype=torch.float64, device=dev)
+    tune = autotune(cs, g, dt)
+    log(f"autotune {tune}")
+    ms, kern_ms = timed_steps(cs, g, T0, Y0, dt, sub, args.steps, args.warmup, stream)
+    stats = counters(cs, g, T0, Y0, dt, sub)
+    kname = f"ie_wave_kernel<{cs.kernel_name.split(':')[1]}>"
+    roof = roofline(tab, stats, kern_ms, kname, f"ie_wave_{args.workload}_{n >> 20}M_pmc.txt")
+    extra = {}
+    if not args.no_consecutive:
+        extra["consecutive_steps"] = consecutive(cs, g, T0, Y0, dt, 3, stream)
+    del cs
+    if args.workload == "syn32" and not args.no_c2:
+        extra["h2air_1M"] = h2air_leg(args)
+    if args.workload == "syn32" and not args.no_surrogate:
+        extra["surrogate"] = surrogate_leg(args)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_legs(mech, _lib.default_opts(**SYN_TOL, jacobian=args.jacobian), st,
+                       args.cpu_sample, args.cpu_single)
+    jac = "reference Jacobian (its dKp/dT sign convention)" if args.jacobian == 0 else \
+        "exact Jacobian"
+    line = {
+        "metric": METRIC, "value": n / (ms * 1e-3), "unit": "cells/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"synthetic {ns}sp/{nr}rxn (seed {seed}), {n:,} cells, "
+                               f"{int(front * 100)}% ignition front, dt 1e-6 s, plain FP64 "
+                               f"ImplicitEuler (Newton rtol 1e-6 atol 1e-10, step error "
+                               f"rtol 1e-3 atol 1e-7), {jac}",
+                   "cells": n, "mechanism": args.workload, "integrator": "ImplicitEuler",
+                   "jacobian": args.jacobian, "parallelism": "single GPU", "n": ns + 1,
+                   "autotune": tune},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+def h2air_run(args, n):
+    """The 1,048,576-cell H2-air workload (compiled lane kernel) with the
+    transport properties at the new state; returns (fields, roofline, context
+    data for the comparison legs)."""
+    import torch
+
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource, Mechanism
+    from openhurricane_amd.sampler import h2air_states
+
+    dev = torch.device("cuda:0")
+    mech = Mechanism.h2air()
+    tab = mech.tables()
+    st = h2air_states(mech.species, tab["W"], n, seed=2024)
+    opts = _lib.default_opts(**H2_TOL, device_pointers=1, schedule=args.schedule,
+                             jacobian=args.jacobian)
+    cs = ChemistrySource(mech, device=0, opts=opts)
+    stream = torch.cuda.current_stream(dev)
+    cs.set_stream(stream.cuda_stream)
+    g = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in st.items()}
+    T0, Y0 = g["T"].clone(), g["Y"].clone()
+    dt = torch.full((n,), 1e-6, dtype=torch.float64, device=dev)
+    sub = torch.zeros(n, dtype=torch.float64, device=dev)
+    tune = autotune(cs, g, dt)
+
+    def props(T, Y):
+        cs.mix_props(T, g["p"], Y, mu=True, kappa=True, Dk=True, cp=False, ha=False)
+
+    ms, kern_ms = timed_steps(cs, g, T0, Y0, dt, sub, args.steps, args.warmup, stream,
+                              after=props)
+    stats = counters(cs, g, T0, Y0, dt, sub)
+    roof = roofline(tab, stats, kern_ms, "ie_lane_kernel<Mech_h2air>",
+                    "ie_lane_h2air_1M_pmc.txt")
+    roof["autotune"] = tune
+    return mech, st, g, T0, Y0, dt, stream, ms, roof
+
+
+def h2air_leg(args):
+    n = 1 << 20
+    mech, st, g, T0, Y0, dt, stream, ms, roof = h2air_run(args, n)
+    out = {"workload": "H2-air 9sp/19rxn, 1,048,576 cells, dt 1e-6 s, plain FP64 ImplicitEuler "
+                       "(rtol 1e-6, atol 1e-10) + mu/kappa/D_k at the new state",
+           "value": n / (ms * 1e-3), "unit": "cells/s", "ms_per_step": ms, "roofline": roof}
+    if not args.no_split:
+        out["split_precision"] = split_precision(args, mech, g, T0, Y0, dt, stream)
+        out["other_integrators"] = other_integrators(mech, g, T0, Y0, dt, stream)
+    return out
+
+
+def bench_h2air(args):
+    from openhurricane_amd import _lib
+
+    n = args.cells or (1 << 20)
+    mech, st, g, T0, Y0, dt, stream, ms, roof = h2air_run(args, n)
+    split = None if args.no_split else split_precision(args, mech, g, T0, Y0, dt, stream)
+    others = None if args.no_split else other_integrators(mech, g, T0, Y0, dt, stream)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_legs(mech, _lib.default_opts(**H2_TOL, jacobian=args.jacobian), st,
+                       args.cpu_sample, args.cpu_single)
+    line = {
+        "metric": METRIC, "value": n / (ms * 1e-3), "unit": "cells/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"H2-air 9sp/19rxn, {n:,} cells, dt 1e-6 s, plain FP64 "
+                               "ImplicitEuler (rtol 1e-6, atol 1e-10) + mu/kappa/D_k",
+                   "cells": n, "mechanism": "h2air", "integrator": "ImplicitEuler",
+                   "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu, "split_precision": split,
+        "other_integrators": others,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def surrogate_leg(args):
+    """Gated surrogate + exact fallback against the plain FP64 integrator on
+    the 32-species workload at its full size: two 34->256^3->33 BF16 networks
+    trained on the GPU on 16,777,216 pairs the GPU integrator generates, then
+    timed and checked on 4,194,304 held-out cells of the same distribution.
+    A failure here is reported in the line, never raised."""
+    pairs, held, epochs = args.surrogate_pairs, args.surrogate_eval, 10
+    try:
+        from openhurricane_amd.surrogate_workflow import run
+
+        r = run(["--pairs", str(pairs), "--eval", str(held), "--epochs", str(epochs),
+                 "--peer-tol", "1e-5", "1e-4", "--jacobian", str(args.jacobian)])
+    except Exception as e:  # the plain-path line must still be printed
+        return {"error": f"{type(e).__name__}: {e}"}
+    gated = []
+    for c in r["gated"]:
+        gated.append({"gate": ("element change <= %g" % c["gate_tol"] if c["gate_tol"] < 1e29
+                               else "none") + ("" if c.get("peer_tol") is None else
+                                               ", peer agreement %g" % c["peer_tol"])
+                      + (", FP8" if c.get("precision") else ""),
+                      "cells_per_s": c["cells_per_s"], "speedup_vs_plain": c["speedup"],
+                      "fallback_frac": c["fallback_frac"],
+                      "accepted_frac_by_kind": c.get("accepted_by_kind"),
+                      "max_rel_err": max(c["max_rel_err_T"], c["max_rel_err_Ymajor"]),
+                      "max_rel_err_T": c["max_rel_err_T"],
+                      "p999_rel_err": c["err"]["p999"]})
+    ok = [c for c in gated if c["max_rel_err"] <= 1e-3]
+    best = max(ok, key=lambda c: c["speedup_vs_plain"]) if ok else None
+    return {"workload": f"synthetic 32sp/150rxn, {pairs:,} training pairs from the GPU "
+                        f"integrator, {held:,} held-out cells (20% ignition front), dt 1e-6 s; "
+                        f"two 34->256^3->33 GELU networks, BF16 on MFMA, {epochs} epochs",
+            "plain_cells_per_s": r["plain"]["cells_per_s"],
+            "plain_seconds_by_kind": r.get("plain_by_kind_seconds"),
+            "network_alone_cells_per_s": r["inference_bf16"]["cells_per_s"],
+            "seconds_generate": r["seconds_generate"], "seconds_train": r["seconds_train"],
+            "train_samples_per_s": r["train_samples_per_s"], "gated": gated,
+            "best_within_1e-3": best,
+            "target": ">= 5x the plain path at <= 1e-3 max relative error (T, Y >= 1e-3)"}
 
 
 def _timed_advance(mech, g, T0, Y0, dt, stream, **kw):
@@ -271,8 +453,7 @@ def other_integrators(mech, g, T0, Y0, dt, stream):
     import torch
 
     n = T0.shape[0]
-    tol = dict(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6)
-    _, Ti, Yi, _ = _timed_advance(mech, g, T0, Y0, dt, stream, **tol)
+    _, Ti, Yi, _ = _timed_advance(mech, g, T0, Y0, dt, stream, **H2_TOL)
 
     def dev(T, Y):
         d = ((T - Ti).abs() / Ti).cpu().numpy()
@@ -288,7 +469,7 @@ def other_integrators(mech, g, T0, Y0, dt, stream):
     out["reference_gpu_explicit_euler"] = dict(
         {"value": n / t, "unit": "cells/s", "steps": st["steps"], "rhs_evals": st["rhs_evals"]},
         **dev(T, Y))
-    t, T, Y, st = _timed_advance(mech, g, T0, Y0, dt, stream, integrator=2, **tol)
+    t, T, Y, st = _timed_advance(mech, g, T0, Y0, dt, stream, integrator=2, **H2_TOL)
     out["implicit_trapezoidal"] = dict(
         {"value": n / t, "unit": "cells/s", "steps": st["steps"],
          "newton_iters": st["newton_iters"]}, **dev(T, Y))
@@ -307,8 +488,7 @@ def split_precision(args, mech, g, T0, Y0, dt, stream):
     from openhurricane_amd import _lib
     from openhurricane_amd.chemistry import ChemistrySource
 
-    kw = dict(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6, device_pointers=1,
-              schedule=args.schedule, jacobian=args.jacobian)
+    kw = dict(H2_TOL, device_pointers=1, schedule=args.schedule, jacobian=args.jacobian)
     res = {}
     for prec in (0, 1):
         c = ChemistrySource(mech, device=0, opts=_lib.default_opts(precision=prec, **kw))
@@ -350,120 +530,40 @@ def split_precision(args, mech, g, T0, Y0, dt, stream):
     return out
 
 
-SYNTH = {"syn32": (32, 150, 1, 1 << 22), "syn54": (54, 270, 2, 1 << 23)}
-
-
-def bench_synthetic(args):
-    """Synthetic-mechanism workloads on the generic wavefront-per-cell kernel:
-    80 % near-equilibrium (half cold, half burnt) and 20 % ignition-front cells
-    (30 % front for the 54-species case), dt = 1e-6 s, ImplicitEuler with
-    Newton tolerances rtol 1e-6 / atol 1e-10 and the reference's default
-    step-error tolerances (RTolIncrement 1e-3, ATolIncrement 1e-7)."""
-    import torch
-
-    from openhurricane_amd import _lib
-    from openhurricane_amd.chemistry import ChemistrySource, Mechanism
-    from openhurricane_amd.flops import integrator_flops, work_flops
-    from openhurricane_amd.sampler import synthetic_workload
-
-    ns, nr, seed, default_cells = SYNTH[args.workload]
-    n = args.cells or default_cells
-    front = 0.2 if args.workload == "syn32" else 0.3
-    mech = Mechanism.synthetic(ns, nr, seed=seed)
-    tab = mech.tables()
-    dev = torch.device("cuda:0")
-    base = dict(atol=1e-10, rtol=1e-6, atol_inc=1e-7, rtol_inc=1e-3, jacobian=args.jacobian)
-    # the burnt-state pool is input preparation: exact Jacobian (fast Newton)
-    burner = ChemistrySource(mech, opts=_lib.default_opts(**dict(base, jacobian=1)))
-
-    def burn(T, p, rho, Y, dt):
-        burner.solve_update(T, p, rho, Y, dt, 1.0)
-
-    print(f"bench: building {n} cells ({args.workload})", file=sys.stderr, flush=True)
-    st = synthetic_workload(mech.species, tab["W"], n, seed=2024, front=front, burn=burn)
-    st.pop("kind")
-    opts = _lib.default_opts(**base, device_pointers=1, schedule=args.schedule)
-    cs = ChemistrySource(mech, device=0, opts=opts)
-    stream = torch.cuda.current_stream(dev)
-    cs.set_stream(stream.cuda_stream)
-    g = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in st.items()}
-    T0, Y0 = g["T"].clone(), g["Y"].clone()
-    T, Y = g["T"].clone(), g["Y"].clone()
-    p, rho = g["p"], g["rho"]
-    dt = torch.full((n,), 1e-6, dtype=torch.float64, device=dev)
-    sub = torch.zeros(n, dtype=torch.float64, device=dev)
-    tune = autotune(cs, g, dt)
-
-    def step(ev=None):
-        T.copy_(T0)
-        Y.copy_(Y0)
-        if ev:
-            ev[0].record(stream)
-        cs.solve_update(T, p, rho, Y, dt, 1.0, subDt=sub, stats=False)
-        if ev:
-            ev[1].record(stream)
-
-    for i in range(args.warmup):
-        step()
-        torch.cuda.synchronize()
-        print(f"bench: warmup step {i} done", file=sys.stderr, flush=True)
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(evs[i])
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    ms = el * 1e3 / args.steps
-    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
-    T.copy_(T0)
-    Y.copy_(Y0)
-    sub_keep = sub.clone()
-    _, stats = cs.solve_update(T, p, rho, Y, dt, 1.0, subDt=sub, stats=True)
-    sub.copy_(sub_keep)
-    fl = work_flops(tab, stats)
-    achieved = fl / (kern_ms * 1e-3) / 1e12
-    N = ns + 1
-    roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-            "kernel": f"ie_wave_kernel<{cs.kernel_name.split(':')[1]}>", "kernel_ms": kern_ms,
-            "autotune": tune,
-            "flops_per_launch": fl, "newton_iters": stats["newton_iters"],
-            "steps": stats["steps"], "flops_per_newton": integrator_flops(tab)["F_newton"]}
-    cpu = None
-    if not args.no_cpu:
-        threads = min(16, len(os.sched_getaffinity(0)))
-        sys.path.insert(0, str(ROOT / "oracle"))
-        from oracle import Oracle
-
-        orc = Oracle(mech)
-        m = args.cpu_sample
-        sl = slice(0, m)
-        Th, ph, rh = (np.ascontiguousarray(st[k][sl]) for k in ("T", "p", "rho"))
-        Yh = np.ascontiguousarray(st["Y"][:, sl])
-        o = _lib.default_opts(**base)
-        _, _, subh, _ = orc.solve_update(o, Th, ph, rh, Yh, np.full(m, 1e-6), 1.0, nthreads=threads)
-        t1 = time.perf_counter()
-        orc.solve_update(o, Th, ph, rh, Yh, np.full(m, 1e-6), 1.0, subDt=subh, nthreads=threads)
-        ec = time.perf_counter() - t1
-        cpu = {"value": m / ec, "unit": "cells/s", "cores": threads, "kind": "port",
-               "sample": f"first {m} cells of the same seeded workload (second flow step), "
-                         f"FP64 oracle, {threads} host threads, {ec:.2f} s wall"}
-    line = {
-        "metric": METRIC, "value": n / (ms * 1e-3), "unit": "cells/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"synthetic {ns}sp/{nr}rxn (seed {seed}), {n:,} cells, "
-                               f"{int(front * 100)}% ignition front, dt 1e-6 s, plain FP64 "
-                               f"ImplicitEuler (Newton rtol 1e-6 atol 1e-10, step error "
-                               f"rtol 1e-3 atol 1e-7), jacobian={args.jacobian}",
-                   "cells": n, "mechanism": args.workload, "integrator": "ImplicitEuler",
-                   "parallelism": "single GPU", "n": N},
-        "roofline": roof, "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="syn32", choices=["syn32", "syn54", "h2air"],
+                    help="syn32 (default): synthetic 32-species/150-reaction, 4,194,304 cells; "
+                         "syn54: 54/270, 8,388,608 cells; h2air: 9-species H2-air, 1,048,576")
+    ap.add_argument("--cells", type=int, default=None, help="cells (default: the workload's own)")
+    ap.add_argument("--cpu-sample", type=int, default=65536,
+                    help="cells of the all-core CPU-baseline sample")
+    ap.add_argument("--cpu-single", type=int, default=2048,
+                    help="cells of the single-core CPU sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--schedule", type=int, default=1,
+                    help="1: longest-first cell order from the previous step's per-cell cost")
+    ap.add_argument("--jacobian", type=int, default=0,
+                    help="0: reference Jacobian (its dKp/dT sign convention); 1: exact")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the gated-surrogate leg")
+    ap.add_argument("--surrogate-pairs", type=int, default=1 << 24)
+    ap.add_argument("--surrogate-eval", type=int, default=1 << 22)
+    ap.add_argument("--no-c2", action="store_true", help="skip the H2-air leg")
+    ap.add_argument("--no-split", action="store_true",
+                    help="skip the FP32 split and the other-integrator comparisons")
+    ap.add_argument("--no-consecutive", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
+              "the chemistry path shards by contiguous cell ranges with no communication, "
+              "see the design notes", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "h2air":
+        return bench_h2air(args)
+    return bench_synthetic(args)
 
 
 if __name__ == "__main__":

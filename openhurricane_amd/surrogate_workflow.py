@@ -25,11 +25,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_context(mech, device_pointers=True):
+def make_context(mech, device_pointers=True, jacobian=1):
     from . import _lib
     from .chemistry import ChemistrySource
 
-    o = _lib.default_opts(atol=1e-10, rtol=1e-6, atol_inc=1e-7, rtol_inc=1e-3, jacobian=1,
+    o = _lib.default_opts(atol=1e-10, rtol=1e-6, atol_inc=1e-7, rtol_inc=1e-3, jacobian=jacobian,
                           device_pointers=int(device_pointers), kernel=2, schedule=1)
     return ChemistrySource(mech, opts=o)
 
@@ -174,9 +174,18 @@ def evaluate(mech, cs, nets, n, seed, front, configs, dt=1e-6, steps=2):
         t, _ = _timed(gated, steps)
         eT, eY = errors(res["gT"], res["gY"], Tr, Yr)
         e = torch.maximum(eT, eY)
+        # a cell that fell back carries the plain integrator's result bit for
+        # bit, an accepted one the network's: acceptance per cell kind
+        acc = (res["gT"] != Tr) | (res["gY"] != Yr).any(0)
+        by_kind = {}
+        for k, name in ((0, "front"), (1, "cold"), (2, "burnt")):
+            sel = kind_t == k
+            if bool(sel.any()):
+                by_kind[name] = float(acc[sel].float().mean())
         r = dict(cfg)
         r.update({"seconds": t, "cells_per_s": n / t, "speedup": t_plain / t,
-                  "fallback_frac": res["nf"] / n, "err": error_stats(e, kind_t),
+                  "fallback_frac": res["nf"] / n, "accepted_by_kind": by_kind,
+                  "err": error_stats(e, kind_t),
                   "max_rel_err_T": float(eT.max()), "max_rel_err_Ymajor": float(eY.max())})
         out["gated"].append(r)
         log(json.dumps(r))
@@ -201,6 +210,9 @@ def run(argv=None):
     ap.add_argument("--peer-tol", type=float, nargs="+", default=[1e-5, 3e-5, 1e-4, 3e-4])
     ap.add_argument("--gate", type=float, default=1e-2, help="element-mass gate")
     ap.add_argument("--save", default=None)
+    ap.add_argument("--jacobian", type=int, default=1,
+                    help="Jacobian of the plain integrator (0: the reference's dKp/dT sign "
+                         "convention, 1: exact)")
     args = ap.parse_args(argv)
 
     import torch
@@ -211,7 +223,7 @@ def run(argv=None):
     ns, nr, seed, width, front = {"syn32": (32, 150, 1, 256, 0.2),
                                   "syn54": (54, 270, 2, 512, 0.3)}[args.mech]
     mech = Mechanism.synthetic(ns, nr, seed=seed)
-    cs = make_context(mech)
+    cs = make_context(mech, jacobian=args.jacobian)
     nets = [Surrogate(cs, width=width, depth=3, seed=k) for k in range(args.nets)]
     delta = None
     if not args.plain_targets:
@@ -241,7 +253,7 @@ def run(argv=None):
     res = evaluate(mech, cs, nets, args.eval, seed=777777, front=front, configs=configs)
     res.update({"mechanism": f"synthetic {ns}sp/{nr}rxn", "pairs": args.pairs,
                 "epochs": args.epochs, "batch": args.batch, "width": width, "depth": 3,
-                "nets": len(nets), "target_scale": None if delta is None else
+                "nets": len(nets), "jacobian": args.jacobian, "target_scale": None if delta is None else
                 {"dT": args.delta_T, "dlnY": args.delta_Y},
                 "loss_history": hist, "seconds_generate": t_gen, "seconds_train": t_fit,
                 "train_samples_per_s": len(nets) * args.pairs * args.epochs / t_fit})
