@@ -30,6 +30,10 @@
  *   ohc_chem_source
  *       chemistrySource::calculateSourceTerms / calculateSourceTermsImp
  *       (chemistrySource.hpp:597-598, chemistrySource.cpp:1371-1450).
+ *   ohc_chem_source_kim
+ *       CUDAChemistrySource::calculateSourceTermsImp / ...ImpAsync / ...ImpAsyncHybrid
+ *       (src/speciesTransport/chemistrySource/src/CUDAChemistrySource.cpp:152-250,
+ *        CUDAChemSourceKimDiagonalKernel.cu:39-235).
  *   ohc_eval_rhs / ohc_eval_jacobian
  *       chemistrySource::DyDt / chemistrySource::jacobian
  *       (chemistrySource.cpp:863-907, :967-1055).
@@ -174,7 +178,20 @@ typedef struct ohc_opts {
                                99-335), the reference's second-order implicit
                                alternative; FP64;
                                3: BDF23 at its default order 2 (BDF23.cpp:
-                               54-509); generic kernels, FP64                  */
+                               54-509); generic kernels, FP64;
+                               4: the reference GPU path's multi-time-scale
+                               explicit Euler (MTSEuler, solveChemODEsMTS,
+                               CUDAChemistrySourceODEs.cu:684-778): species
+                               grouped by the decade of their time scale,
+                               group steps of 0.8 x its smallest time scale,
+                               atol/rtol the group's increment test; every
+                               cell integrated, subDt left as it is, Y from c
+                               without clipping; FP64                          */
+    int64_t max_slice_cells;/* host-pointer calls: cells staged on the device at
+                               a time (memory-driven slicing); 0 = as many as
+                               half the free device memory holds.  Results do
+                               not depend on it; the longest-first schedule
+                               is not used within a sliced call              */
 } ohc_opts;
 
 /* Per-call counters (summed over cells) for roofline accounting. */
@@ -253,6 +270,19 @@ int ohc_chem_solve_update(ohc_ctx *ctx, int64_t n, double *T, const double *p,
 int ohc_chem_source(ohc_ctx *ctx, int64_t n, const double *T, const double *p,
                     const double *rho, const double *Y, int64_t ldY, double *R,
                     int64_t ldR, double *diagJ, int64_t ldD, int32_t withoutLastSpec);
+
+/* The GPU path's calculateSourceTermsImp / calculateSourceTermsImpAsync /
+ * calculateSourceTermsImpAsyncHybrid (the "Kim" diagonal kernel): R_k =
+ * omega_k W_k (FP64, ns rows of ldR) and, into diag (nullable, ns-1 rows of
+ * ldD), the mass-action diagonal d(omega_k)/d(c_k) of species k < ns-1 with the
+ * last species eliminated by mass conservation (d c_ns / d c_k = -W_k / W_ns);
+ * the third-body and fall-off factors of kf and kr are held fixed and there is
+ * no temperature coupling.  diag_fp32 = 0: diag is double*; 1: float* (the
+ * Hybrid variant).  Needs the run-time device tables (ns <= 63, at most 3
+ * distinct species per reaction side). */
+int ohc_chem_source_kim(ohc_ctx *ctx, int64_t n, const double *T, const double *p,
+                        const double *rho, const double *Y, int64_t ldY, double *R, int64_t ldR,
+                        void *diag, int64_t ldD, int32_t diag_fp32);
 
 /* RHS and analytic Jacobian at ODE states y = [c_1..c_ns, T] (ld = row stride of
  * the (ns+1) x n block); J is (ns+1)^2 x n, entry (i,j) of cell c at

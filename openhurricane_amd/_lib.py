@@ -46,7 +46,7 @@ class Opts(C.Structure):
         ("T_beta", C.c_double), ("dt_max", C.c_double), ("T_low", C.c_double),
         ("T_high", C.c_double), ("precision", C.c_int32), ("device_pointers", C.c_int32),
         ("kernel", C.c_int32), ("waves_per_cu", C.c_int32), ("schedule", C.c_int32),
-        ("jacobian", C.c_int32), ("integrator", C.c_int32),
+        ("jacobian", C.c_int32), ("integrator", C.c_int32), ("max_slice_cells", C.c_int64),
     ]
 
 
@@ -95,6 +95,8 @@ _SIGS = [
                                 _V, _V, _V]),
     ("ohc_thermo_T_from_e", C.c_int, [_V, C.c_int64, _V, _V, _V, C.c_int64, _V, _V]),
     ("ohc_mix_thermo", C.c_int, [_V, C.c_int64, _V, _V, _V, C.c_int64, _V, _V, _V, _V]),
+    ("ohc_chem_source_kim", C.c_int, [_V, C.c_int64, _V, _V, _V, _V, C.c_int64, _V, C.c_int64,
+                                      _V, C.c_int64, C.c_int32]),
 ]
 
 _FP = C.POINTER(C.c_float)

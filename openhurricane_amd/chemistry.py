@@ -308,6 +308,26 @@ class ChemistrySource:
         check(rc, self._h)
         return (R, D) if diag else R
 
+    # the GPU path's calculateSourceTermsImp[Async][Hybrid]: R [ns, n] and the
+    # substituted mass-action diagonal [ns-1, n] (float32 with fp32=True)
+    def source_terms_kim(self, T, p, rho, Y, fp32=False):
+        n = T.shape[0]
+        R = self._zeros((self.ns, n))
+        if self.device_pointers:
+            import torch
+
+            D = torch.zeros((self.ns - 1, n), dtype=torch.float32 if fp32 else torch.float64,
+                            device=f"cuda:{self.device}")
+            dptr = C.c_void_p(D.data_ptr())
+        else:
+            D = np.zeros((self.ns - 1, n), dtype=np.float32 if fp32 else np.float64)
+            dptr = C.c_void_p(D.ctypes.data)
+        rc = lib().ohc_chem_source_kim(self._h, n, self._p(T), self._p(p), self._p(rho),
+                                       self._p(Y), _ld(Y, n), self._p(R), _ld(R, n), dptr, n,
+                                       int(fp32))
+        check(rc, self._h)
+        return R, D
+
     def mix_props(self, T, p, Y, mu=True, kappa=True, Dk=True, cp=True, ha=True, hk=False):
         n = T.shape[0]
         out = {}

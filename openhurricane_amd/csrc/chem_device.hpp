@@ -545,7 +545,10 @@ __device__ __forceinline__ FallOff<M, r> falloff(const TFac &tf, double Mtb) {
 // net rates, Jacobian, temperature equation) in FP64.
 // UNI: when every lane of the wavefront is above every Tcommon, evaluate the
 // thermo with the high coefficient sets as constants (else per-lane selects).
-template <class M, bool WITH_J, int FLOW, class F = double, bool UNI = false>
+// MA: mass-action Jacobian only -- the third-body and fall-off q dln(gamma)/dc
+// terms are left out, so that J[k][k] is reactionTable::dWdciAllSpecies's
+// diagonal (the MTS time scales).
+template <class M, bool WITH_J, int FLOW, class F = double, bool UNI = false, bool MA = false>
 __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
                                         double (&f)[M::NS + 1],
                                         double (&J)[M::NS + 1][M::NS + 1], bool exact = false) {
@@ -651,7 +654,8 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
                 sfor<0, d.nb>([&](auto j) { J[d.bi[j]][si] -= d.bnu[j] * dq; });
             });
             // third-body terms q * dln(gamma)/dc_i
-            if constexpr (d.kind == 1) {
+            if constexpr (MA) {
+            } else if constexpr (d.kind == 1) {
                 // J(a, i) += sigma_a q eff_i / max(M, tiny): accumulate sigma_a q / M per
                 // row, the eff_i outer product is applied once per group below.
                 const double w = q * rMg[d.tb];
@@ -705,7 +709,7 @@ __device__ __forceinline__ void eval_fJ(const double (&y)[M::NS + 1], double pr,
             sfor<0, d.nb>([&](auto i) { J[d.bi[i]][NS] += d.bnu[i] * s; });
         }
     });
-    if constexpr (WITH_J) {
+    if constexpr (WITH_J && !MA) {
         sfor<0, M::NTB>([&](auto g) {
             sfor<0, NS>([&](auto a) {
                 if constexpr (M::TBROW[g][a]) {

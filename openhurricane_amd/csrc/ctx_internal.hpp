@@ -38,9 +38,9 @@ struct ohc_ctx {
     // longest-first schedule: per-cell cost of the last call and its order
     int64_t sched_n = 0;
     bool sched_ready = false;
-    int *d_cost = nullptr, *d_order = nullptr, *d_iota = nullptr, *d_keys = nullptr;
-    void *d_sort_tmp = nullptr;
-    size_t sort_tmp_bytes = 0;
+    int *d_cost = nullptr, *d_order = nullptr;
+    unsigned *d_bins = nullptr;  // counting-sort buckets (2 * ohcs::kBins)
+    bool in_slices = false;  // inside a sliced host-pointer call (one slice at a time)
     std::string err, kname;
 };
 
@@ -146,7 +146,26 @@ class Staging {
 };
 
 int check_ctx(ohc_ctx *c);
+// Memory-driven slicing of host-pointer calls: true (and the cells per slice
+// in *m) when n cells of bytes_per_cell staged bytes each are to be done in
+// slices -- opts.max_slice_cells, or as many cells as half the free device
+// memory holds (turbulentSpeciesSolverCUDATran.cpp:196-222 splits into
+// nSlice = used / available + 1).  Device-pointer calls are never sliced.
+bool host_slices(ohc_ctx *c, int64_t n, size_t bytes_per_cell, int64_t *m);
+template <class T>
+T *off(T *p, int64_t s) {
+    return p ? p + s : nullptr;
+}
 int read_stats(ohc_ctx *c, ohc_stats *st);
+// Counters of one integrator launch (the 8 stats words the kernels write:
+// cells, steps, attempts, Newton iterations, RHS evaluations, max cell steps,
+// dt_min bits, failed cells) into st (nullable); failed cells -> error.
+int stats_from_words(ohc_ctx *c, const unsigned long long *words, ohc_stats *st);
+// Integrator parameters that come from the context options (pointers, n and
+// the queue left zero).
+void base_params(const ohc_ctx *c, ohcd::LaneParams &P);
+// Launch the context's integrator kernel (lane or generic) on its stream.
+hipError_t launch_integrator(const ohc_ctx *c, const ohcd::LaneParams &P);
 // chemistrySource::solve (mode 0) / solveUpdate (mode 1) over n cells; with a
 // device list of cell indices only those nlist cells are integrated.
 int chem_integrate(ohc_ctx *c, int mode, int64_t n, const double *T, double *Tio,

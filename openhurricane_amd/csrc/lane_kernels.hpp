@@ -725,6 +725,177 @@ __global__ __launch_bounds__(64) void ee_lane_kernel(const LaneParams P) {
     }
 }
 
+// MTS_whichGroup: 1 + decades of tn / tci (1 when tci >= tn)
+__device__ __forceinline__ int mts_group(double tn, double tci) {
+    if (!(tn > tci)) return 1;
+    const int g = (int)log10(tn / tci);
+    return (g > 0 ? g : 0) + 1;
+}
+
+// ---- MTS explicit Euler (the reference GPU path's MTSEuler solver type:
+// solveChemODEsMTS, CUDAChemistrySourceODEs.cu:684-778, timescale /
+// MTS_whichGroup / MTS_GroupDT :199-309): the algorithm of mts_wave_kernel
+// with one cell per lane and one RHS evaluation per lane per loop trip.  The
+// per-cell state lives in LDS (one slot per lane), like ee_lane_kernel's.
+template <class M, int FLOW>
+__global__ __launch_bounds__(64) void mts_lane_kernel(const LaneParams P) {
+    constexpr int NS = M::NS, N = NS + 1;
+    enum { T_NEED, T_STEP, T_DONE };
+    const int lane = threadIdx.x;
+    __shared__ double s_y[N][64], s_tci[NS][64];
+    __shared__ int s_ig[NS][64];
+    enum { C_PR, C_TN, C_DID, C_DTG, C_DTMIN, C_NCTL };
+    enum { S_CELLS, S_STEPS, S_MAXS, S_FAIL, S_CELL, S_NST, S_IM, S_NM, S_NCNT };
+    __shared__ double s_c[C_NCTL][64];
+    __shared__ long long s_n[S_NCNT][64];
+#define CT(i) s_c[i][lane]
+#define CNT(i) s_n[i][lane]
+#pragma unroll
+    for (int i = 0; i < C_NCTL; ++i) CT(i) = 0.0;
+#pragma unroll
+    for (int i = 0; i < S_NCNT; ++i) CNT(i) = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) s_y[i][lane] = 0.0;
+    s_y[NS][lane] = 1000.0;
+    CT(C_DTMIN) = __builtin_huge_val();
+    __syncthreads();
+    int phase = T_NEED;
+    auto finish_cell = [&](bool failed) __attribute__((always_inline)) {
+        const long long cell = CNT(S_CELL);
+        const double tn = CT(C_TN);
+        if (P.mode == 0) {
+            const double rho = P.rho[cell], rdt = 1.0 / tn;
+            sfor<0, NS>([&](auto k) {
+                const double c0 = rho * P.Y[k * P.ldY + cell] / M::W[k];
+                P.R[k * P.ldR + cell] = (s_y[k][lane] - c0) * rdt * M::W[k];
+            });
+        } else {
+            // device ciToYi (CUDAChemistrySourceODEs.cu:545-586): Y = c W / sum c W
+            double r = 0.0;
+            sfor<0, NS>([&](auto k) { r += s_y[k][lane] * M::W[k]; });
+            const double rr = 1.0 / r;
+            sfor<0, NS>([&](auto k) { P.Yout[k * P.ldY + cell] = s_y[k][lane] * M::W[k] * rr; });
+            P.Tout[cell] = s_y[NS][lane];
+        }
+        const long long cs = CNT(S_NST);
+        if (P.cost) P.cost[cell] = (int)(cs < 0x7fffffffll ? cs : 0x7fffffffll);
+        CNT(S_CELLS) += 1;
+        CNT(S_STEPS) += cs;
+        if (failed) CNT(S_FAIL) += 1;
+        if (cs > CNT(S_MAXS)) CNT(S_MAXS) = cs;
+        CT(C_DTMIN) = fmin(CT(C_DTMIN), P.subDt[cell]);
+        phase = T_NEED;
+    };
+    // the next group with species after im (Nm when none), and its step
+    auto next_group = [&](long long im) __attribute__((always_inline)) {
+        long long nx = CNT(S_NM);
+        sfor<0, NS>([&](auto k) {
+            const long long g = s_ig[k][lane];
+            if (g > im && g < nx) nx = g;
+        });
+        double dtG = __builtin_huge_val();
+        sfor<0, NS>([&](auto k) {
+            if (s_ig[k][lane] == nx) dtG = fmin(dtG, s_tci[k][lane]);
+        });
+        CNT(S_IM) = nx;
+        CT(C_DTG) = dtG;
+        return nx < CNT(S_NM);
+    };
+    for (;;) {
+        for (;;) {
+            const unsigned long long need = __ballot(phase == T_NEED);
+            if (need == 0ull) break;
+            const int leader = __ffsll((long long)need) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(P.counter, (unsigned long long)__popcll(need));
+            base = __shfl(base, leader);
+            if (phase == T_NEED) {
+                const unsigned long long below = lane ? (need & ((1ull << lane) - 1ull)) : 0ull;
+                const unsigned long long mine = base + (unsigned long long)__popcll(below);
+                if (mine >= (unsigned long long)P.n) {
+                    phase = T_DONE;
+                } else {
+                    const long long cell = P.order ? (long long)P.order[mine] : (long long)mine;
+                    CNT(S_CELL) = cell;
+                    const double rho = P.rho[cell];
+                    const double pr = FLOW == 0 ? P.p[cell] : rho;
+                    const double tn = P.dt[cell] * P.dtFactor;
+                    CT(C_PR) = pr;
+                    CT(C_TN) = tn;
+                    CT(C_DID) = 0.0;
+                    CNT(S_NST) = 0;
+                    double y[N], f[N], J[N][N];
+                    sfor<0, NS>([&](auto k) { y[k] = rho * P.Y[k * P.ldY + cell] / M::W[k]; });
+                    y[NS] = P.T[cell];
+                    sfor<0, N>([&](auto i) { s_y[i][lane] = y[i]; });
+                    // timescale / MTS_whichGroup / MTS_GroupDT
+                    eval_fJ<M, true, FLOW, double, false, true>(y, pr, f, J);
+                    double tmin = __builtin_huge_val();
+                    sfor<0, NS>([&](auto k) {
+                        const double t = fmin(200.0, 1.0 / (fabs(J[k][k]) + 1e-20));
+                        s_tci[k][lane] = t;
+                        tmin = fmin(tmin, t);
+                    });
+                    const int Nm = tn > tmin ? (int)fmax(0.0, floor(log10(tn / tmin))) + 1 : 1;
+                    CNT(S_NM) = Nm;
+                    sfor<0, NS>([&](auto k) { s_ig[k][lane] = Nm - mts_group(tn, s_tci[k][lane]); });
+                    if (next_group(-1) && tn > 0.0) phase = T_STEP;
+                    else finish_cell(false);
+                }
+            }
+        }
+        if (!__any(phase == T_STEP)) break;
+        double y[N], f[N], Jd[N][N];
+        sfor<0, N>([&](auto i) { y[i] = s_y[i][lane]; });
+        eval_fJ<M, false, FLOW>(y, CT(C_PR), f, Jd);
+        if (phase == T_STEP) {
+            const long long im = CNT(S_IM), Nm = CNT(S_NM);
+            if (CNT(S_NST) >= P.max_steps) {
+                finish_cell(true);
+            } else {
+                const double tn = CT(C_TN), did = CT(C_DID);
+                double dt = 0.8 * CT(C_DTG);
+                if (did + dt > tn) dt = tn - did;
+                const double atol = P.atol, rtol = P.rtol;
+                double e = 0.0;
+                sfor<0, N>([&](auto i) {
+                    double fi = f[i];
+                    if constexpr (i < NS) {
+                        if (s_ig[i][lane] < im) fi = 0.0;  // faster groups are frozen
+                    }
+                    const double err = dt * fi, ay0 = fabs(y[i]);
+                    const double yn = y[i] + err;
+                    s_y[i][lane] = yn;
+                    if constexpr (i < NS) {
+                        const double tol = atol + rtol * fmax(fabs(yn), ay0);
+                        if (s_ig[i][lane] == im) e = fmax(e, fabs(err) / tol);
+                    }
+                });
+                const double nd = did + dt;
+                CT(C_DID) = nd;
+                CNT(S_NST) += 1;
+                if (!(nd < tn)) finish_cell(false);
+                else if (im != Nm - 1 && e < 1.0 && !next_group(im)) finish_cell(false);
+            }
+        }
+    }
+    const long long s0 = wave_sum(CNT(S_CELLS)), s1 = wave_sum(CNT(S_STEPS)),
+                    s5 = wave_max(CNT(S_MAXS)), s7 = wave_sum(CNT(S_FAIL));
+    const double dmin = wave_min(CT(C_DTMIN));
+#undef CT
+#undef CNT
+    if (lane == 0) {
+        atomicAdd(&P.stats[0], (unsigned long long)s0);
+        atomicAdd(&P.stats[1], (unsigned long long)s1);
+        atomicAdd(&P.stats[2], (unsigned long long)s1);
+        atomicAdd(&P.stats[4], (unsigned long long)s1);
+        atomicMax(&P.stats[5], (unsigned long long)s5);
+        if (dmin < __builtin_huge_val() && dmin >= 0.0)
+            atomicMin(&P.stats[6], (unsigned long long)__double_as_longlong(dmin));
+        atomicAdd(&P.stats[7], (unsigned long long)s7);
+    }
+}
+
 // ---- RHS / Jacobian evaluation at given states (chemistrySource::DyDt / jacobian) ----
 struct EvalParams {
     int64_t n;
@@ -1022,7 +1193,10 @@ struct LaneEntry {
 template <class M>
 struct LaneLaunch {
     static hipError_t integrate(const LaneParams &p, int grid, hipStream_t s) {
-        if (p.integrator == 1) {
+        if (p.integrator == 4) {
+            if (p.flow == 0) hipLaunchKernelGGL((mts_lane_kernel<M, 0>), dim3(grid), dim3(64), 0, s, p);
+            else hipLaunchKernelGGL((mts_lane_kernel<M, 1>), dim3(grid), dim3(64), 0, s, p);
+        } else if (p.integrator == 1) {
             if (p.flow == 0) hipLaunchKernelGGL((ee_lane_kernel<M, 0>), dim3(grid), dim3(64), 0, s, p);
             else hipLaunchKernelGGL((ee_lane_kernel<M, 1>), dim3(grid), dim3(64), 0, s, p);
         } else if (p.integrator == 2) {

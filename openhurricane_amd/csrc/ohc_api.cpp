@@ -5,7 +5,6 @@
 // host<->device staging for host-pointer calls and kernel dispatch.  Every
 // entry point returns a status; nothing aborts.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -20,6 +19,7 @@
 #include "../../include/ohc.h"
 #include "lane_kernels.hpp"
 #include "mech.hpp"
+#include "sched.hpp"
 #include "wave_dispatch.hpp"
 #include "wave_kernels.hpp"
 
@@ -55,18 +55,33 @@ int check_ctx(ohc_ctx *c) {
     return OHC_OK;
 }
 
-__global__ void iota_kernel(int *a, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = (int)i;
+bool host_slices(ohc_ctx *c, int64_t n, size_t bytes_per_cell, int64_t *m) {
+    if (c->opts.device_pointers || c->in_slices || n <= 0) return false;
+    int64_t cells = c->opts.max_slice_cells;
+    if (cells <= 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr == 0) return false;
+        cells = (int64_t)(fr / 2 / std::max<size_t>(bytes_per_cell, 1));
+    }
+    *m = std::max<int64_t>(cells, 1);
+    return n > *m;
+}
+
+// Runs f(s, k) over the slices [s, s + k) of n cells with c->in_slices set.
+template <class F>
+int run_slices(ohc_ctx *c, int64_t n, int64_t m, F &&f) {
+    c->in_slices = true;
+    int rc = OHC_OK;
+    for (int64_t s = 0; s < n && rc == OHC_OK; s += m) rc = f(s, std::min(m, n - s));
+    c->in_slices = false;
+    return rc;
 }
 
 void free_sched(ohc_ctx *c) {
-    for (void *p : {(void *)c->d_cost, (void *)c->d_order, (void *)c->d_iota, (void *)c->d_keys,
-                    c->d_sort_tmp})
+    for (void *p : {(void *)c->d_cost, (void *)c->d_order, (void *)c->d_bins})
         if (p) (void)hipFree(p);
-    c->d_cost = c->d_order = c->d_iota = c->d_keys = nullptr;
-    c->d_sort_tmp = nullptr;
-    c->sort_tmp_bytes = 0;
+    c->d_cost = c->d_order = nullptr;
+    c->d_bins = nullptr;
     c->sched_n = 0;
     c->sched_ready = false;
 }
@@ -80,17 +95,7 @@ int ensure_sched(ohc_ctx *c, int64_t n) {
     const size_t b = sizeof(int) * (size_t)n;
     HIPCHK(c, hipMalloc((void **)&c->d_cost, b));
     HIPCHK(c, hipMalloc((void **)&c->d_order, b));
-    HIPCHK(c, hipMalloc((void **)&c->d_iota, b));
-    HIPCHK(c, hipMalloc((void **)&c->d_keys, b));
-    size_t tmp = 0;
-    HIPCHK(c, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, c->d_cost, c->d_keys,
-                                                           c->d_iota, c->d_order, (int)n, 0,
-                                                           32, c->stream));
-    HIPCHK(c, hipMalloc(&c->d_sort_tmp, tmp));
-    c->sort_tmp_bytes = tmp;
-    hipLaunchKernelGGL(iota_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                       c->d_iota, n);
-    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMalloc((void **)&c->d_bins, sizeof(unsigned) * 2 * ohcs::kBins));
     c->sched_n = n;
     c->sched_ready = false;
     return OHC_OK;
@@ -217,20 +222,61 @@ int read_stats(ohc_ctx *c, ohc_stats *st) {
     unsigned long long h[9];
     HIPCHK(c, hipMemcpyAsync(h, c->d_ctl, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (st) {
-        st->cells = (int64_t)h[1];
-        st->steps = (int64_t)h[2];
-        st->attempts = (int64_t)h[3];
-        st->newton_iters = (int64_t)h[4];
-        st->rhs_evals = (int64_t)h[5];
-        st->max_cell_steps = (int64_t)h[6];
-        double d;
-        std::memcpy(&d, &h[7], sizeof(d));
-        st->dt_min = h[1] ? d : 1.0e15;
+    return stats_from_words(c, h + 1, st);
+}
+
+void base_params(const ohc_ctx *c, ohcd::LaneParams &P) {
+    std::memset(&P, 0, sizeof(P));
+    const ohc_opts &o = c->opts;
+    P.flow = o.flow_type;
+    P.max_steps = o.max_steps;
+    P.atol = o.atol;
+    P.rtol = o.rtol;
+    P.atol_inc = o.atol_inc;
+    P.rtol_inc = o.rtol_inc;
+    P.safe = o.safe_scale;
+    P.minsc = o.min_scale;
+    P.maxsc = o.max_scale;
+    P.alpha = o.alpha_inc;
+    P.Tthr = o.T_threshold;
+    P.Talpha = o.T_alpha;
+    P.Tbeta = o.T_beta;
+    P.dt_max = o.dt_max;
+    P.max_cell_iters = 50000000;
+    P.exact_dkp = o.jacobian;
+    P.prec = o.precision;
+    P.integrator = o.integrator;
+}
+
+hipError_t launch_integrator(const ohc_ctx *c, const ohcd::LaneParams &P) {
+    if (P.n <= 0) return hipSuccess;
+    if (c->lane) {
+        const int grid = (int)std::min<int64_t>(c->grid, (P.n + 63) / 64);
+        return c->lane->integrate(P, grid, c->stream);
     }
-    if (h[8]) {
+    ohcd::WaveParams W;
+    W.L = P;
+    W.M = c->wm;
+    W.exact_dkp = c->opts.jacobian;
+    const int grid = (int)std::min<int64_t>(c->grid, P.n);
+    return ohcd::wave_integrate(W, grid, c->stream);
+}
+
+int stats_from_words(ohc_ctx *c, const unsigned long long *h, ohc_stats *st) {
+    if (st) {
+        st->cells = (int64_t)h[0];
+        st->steps = (int64_t)h[1];
+        st->attempts = (int64_t)h[2];
+        st->newton_iters = (int64_t)h[3];
+        st->rhs_evals = (int64_t)h[4];
+        st->max_cell_steps = (int64_t)h[5];
+        double d;
+        std::memcpy(&d, &h[6], sizeof(d));
+        st->dt_min = h[0] ? d : 1.0e15;
+    }
+    if (h[7]) {
         return fail(c, OHC_ERR_NUMERIC,
-                    std::to_string(h[8]) +
+                    std::to_string(h[7]) +
                         " cell(s) failed: step size underflow (dt <= 1e-300) or iteration cap");
     }
     return OHC_OK;
@@ -259,6 +305,33 @@ int chem_integrate(ohc_ctx *c, int mode, int64_t n, const double *T, double *Tio
     if (!(dtFactor > 0)) return fail(c, OHC_ERR_ARG, "dtFactor must be > 0");
     if (!c->lane && !c->wave)
         return fail(c, OHC_ERR_UNSUPPORTED, "no device integrator for this mechanism");
+    int64_t m = 0;
+    if (!list && host_slices(c, n, sizeof(double) * (5 + 2 * (size_t)ns), &m)) {
+        ohc_stats acc;
+        std::memset(&acc, 0, sizeof(acc));
+        acc.dt_min = 1.0e15;
+        rc = run_slices(c, n, m, [&](int64_t s, int64_t k) {
+            ohc_stats one;
+            const int r = chem_integrate(c, mode, k, off(T, s), off(Tio, s), off(p, s),
+                                         off(rho, s), off(Y, s), off(Yio, s), ldY, off(dt, s),
+                                         dtFactor, off(subDt, s), off(R, s), ldR, &one, nullptr, 0);
+            if (r) return r;
+            acc.cells += one.cells;
+            acc.steps += one.steps;
+            acc.attempts += one.attempts;
+            acc.newton_iters += one.newton_iters;
+            acc.rhs_evals += one.rhs_evals;
+            acc.max_cell_steps = std::max(acc.max_cell_steps, one.max_cell_steps);
+            acc.dt_min = std::min(acc.dt_min, one.dt_min);
+            return OHC_OK;
+        });
+        if (rc) return rc;
+        if (stats) *stats = acc;
+        return OHC_OK;
+    }
+    // the longest-first schedule needs the same cells call after call: not
+    // within slices
+    const bool sched = c->opts.schedule && !list && !c->in_slices;
     const size_t sv = sizeof(double) * (size_t)n;
     Staging st(c);
     void *dT, *dp, *drho, *dY, *ddt, *dsub, *dR = nullptr;
@@ -274,7 +347,7 @@ int chem_integrate(ohc_ctx *c, int mode, int64_t n, const double *T, double *Tio
         if ((rc = st.map_rows(R, ns, n, ldR, sizeof(double), false, true, &dR))) return rc;
     }
     ohcd::LaneParams P;
-    std::memset(&P, 0, sizeof(P));
+    base_params(c, P);
     const ohc_opts &o = c->opts;
     P.n = n;
     P.T = (const double *)dT;
@@ -290,56 +363,23 @@ int chem_integrate(ohc_ctx *c, int mode, int64_t n, const double *T, double *Tio
     P.Tout = (double *)dT;
     P.Yout = (double *)dY;
     P.mode = mode;
-    P.flow = o.flow_type;
-    P.max_steps = o.max_steps;
-    P.atol = o.atol;
-    P.rtol = o.rtol;
-    P.atol_inc = o.atol_inc;
-    P.rtol_inc = o.rtol_inc;
-    P.safe = o.safe_scale;
-    P.minsc = o.min_scale;
-    P.maxsc = o.max_scale;
-    P.alpha = o.alpha_inc;
-    P.Tthr = o.T_threshold;
-    P.Talpha = o.T_alpha;
-    P.Tbeta = o.T_beta;
-    P.dt_max = o.dt_max;
-    P.max_cell_iters = 50000000;
     if (list) {
         // integrate only the listed cells (device array of cell indices)
         P.n = nlist;
         P.order = list;
-    } else if (o.schedule) {
+    } else if (sched) {
         if ((rc = ensure_sched(c, n))) return rc;
         P.order = c->sched_ready ? c->d_order : nullptr;
         P.cost = c->d_cost;
     }
     P.counter = c->d_ctl;
     P.stats = c->d_ctl + 1;
-    P.exact_dkp = o.jacobian;
-    P.prec = o.precision;
-    P.integrator = o.integrator;
     hipLaunchKernelGGL(reset_ctl, dim3(1), dim3(64), 0, c->stream, c->d_ctl);
     HIPCHK(c, hipGetLastError());
-    if (P.n > 0) {
-        if (c->lane) {
-            const int grid = (int)std::min<int64_t>(c->grid, (P.n + 63) / 64);
-            HIPCHK(c, c->lane->integrate(P, grid, c->stream));
-        } else {
-            ohcd::WaveParams W;
-            W.L = P;
-            W.M = c->wm;
-            W.exact_dkp = o.jacobian;
-            const int grid = (int)std::min<int64_t>(c->grid, P.n);
-            HIPCHK(c, ohcd::wave_integrate(W, grid, c->stream));
-        }
-    }
-    if (o.schedule && !list) {
+    HIPCHK(c, launch_integrator(c, P));
+    if (sched) {
         // next call on these cells: most expensive first
-        size_t tmp = c->sort_tmp_bytes;
-        HIPCHK(c, hipcub::DeviceRadixSort::SortPairsDescending(c->d_sort_tmp, tmp, c->d_cost,
-                                                               c->d_keys, c->d_iota, c->d_order,
-                                                               (int)n, 0, 32, c->stream));
+        HIPCHK(c, ohcs::order_by_cost(c->d_cost, c->d_order, c->d_bins, n, c->stream));
         c->sched_ready = true;
     }
     if ((rc = st.finish())) return rc;
@@ -372,6 +412,7 @@ int ohc_opts_default(ohc_opts *o) {
     o->dt_max = 1.0e15;
     o->T_low = 50.0;
     o->T_high = 5000.0;
+    o->max_slice_cells = 0;
     return OHC_OK;
 }
 
@@ -492,9 +533,11 @@ int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_
     if (!c->lane && o.precision == 1)
         return fail(nullptr, OHC_ERR_UNSUPPORTED,
                     "the FP32 split precision mode needs a specialised kernel for this mechanism");
-    if (o.integrator < 0 || o.integrator > 3) return fail(nullptr, OHC_ERR_ARG, "integrator must be 0, 1, 2 or 3");
+    if (o.integrator < 0 || o.integrator > 4)
+        return fail(nullptr, OHC_ERR_ARG, "integrator must be 0, 1, 2, 3 or 4");
     if (o.integrator >= 2 && o.precision != 0)
-        return fail(nullptr, OHC_ERR_UNSUPPORTED, "the ImplicitTrapezoidal and BDF23 integrators run in FP64 only");
+        return fail(nullptr, OHC_ERR_UNSUPPORTED,
+                    "the ImplicitTrapezoidal, BDF23 and MTS integrators run in FP64 only");
     if (o.integrator == 3 && o.kernel == 1)
         return fail(nullptr, OHC_ERR_UNSUPPORTED, "the BDF23 integrator runs on the generic kernels");
     if (o.integrator == 1 && (!c->lane || o.precision != 0))
@@ -508,6 +551,12 @@ int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_
         int rc = build_wmech(cp);
         if (rc) return fail(nullptr, rc, cp->err);
         c->wave = true;
+    } else if (build_wmech(cp) != OHC_OK) {
+        // a compiled mechanism the run-time tables cannot hold: only the
+        // entry points that need them (ohc_chem_source_kim) are unavailable
+        if (c->d_wm) (void)hipFree(c->d_wm);
+        c->d_wm = nullptr;
+        cp->err.clear();
     }
     {
         hipDeviceProp_t prop;
@@ -680,6 +729,12 @@ int ohc_chem_source(ohc_ctx *c, int64_t n, const double *T, const double *p, con
     if (n == 0) return OHC_OK;
     if (!c->lane && !c->wave) return fail(c, OHC_ERR_UNSUPPORTED, "no device kernels for this mechanism");
     if (!c->lane && n > 0x7fffffffll) return fail(c, OHC_ERR_ARG, "n too large");
+    int64_t m = 0;
+    if (host_slices(c, n, sizeof(double) * (3 + 3 * (size_t)ns), &m))
+        return run_slices(c, n, m, [&](int64_t s, int64_t k) {
+            return ohc_chem_source(c, k, T + s, p + s, rho + s, Y + s, ldY, R + s, ldR,
+                                   off(diagJ, s), ldD, withoutLastSpec);
+        });
     Staging st(c);
     const size_t sv = sizeof(double) * (size_t)n;
     void *dT, *dp, *drho, *dY, *dR, *dD = nullptr;
@@ -699,6 +754,45 @@ int ohc_chem_source(ohc_ctx *c, int64_t n, const double *T, const double *p, con
     return st.finish();
 }
 
+int ohc_chem_source_kim(ohc_ctx *c, int64_t n, const double *T, const double *p,
+                        const double *rho, const double *Y, int64_t ldY, double *R, int64_t ldR,
+                        void *diag, int64_t ldD, int32_t diag_fp32) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    const int ns = c->mech.ns;
+    if (n < 0 || ldY < n || ldR < n || (diag && ldD < n)) return fail(c, OHC_ERR_ARG, "bad sizes");
+    if (!T || !p || !rho || !Y || !R) return fail(c, OHC_ERR_ARG, "null array argument");
+    if (diag_fp32 != 0 && diag_fp32 != 1) return fail(c, OHC_ERR_ARG, "diag_fp32 must be 0 or 1");
+    if (ns < 2) return fail(c, OHC_ERR_UNSUPPORTED, "the substituted diagonal needs ns >= 2");
+    if (n == 0) return OHC_OK;
+    if (!c->d_wm)
+        return fail(c, OHC_ERR_UNSUPPORTED,
+                    "no run-time device tables for this mechanism (ns <= 63 and at most 3 "
+                    "distinct species per reaction side)");
+    if (n > 0x7fffffffll) return fail(c, OHC_ERR_ARG, "n too large");
+    const size_t esz = diag_fp32 ? sizeof(float) : sizeof(double);
+    int64_t m = 0;
+    if (host_slices(c, n, sizeof(double) * (3 + 2 * (size_t)ns) + esz * (size_t)ns, &m))
+        return run_slices(c, n, m, [&](int64_t s, int64_t k) {
+            return ohc_chem_source_kim(c, k, T + s, p + s, rho + s, Y + s, ldY, R + s, ldR,
+                                       diag ? (void *)((char *)diag + esz * (size_t)s) : nullptr,
+                                       ldD, diag_fp32);
+        });
+    Staging st(c);
+    const size_t sv = sizeof(double) * (size_t)n;
+    void *dT, *dp, *drho, *dY, *dR, *dD = nullptr;
+    if ((rc = st.map(T, sv, true, false, &dT))) return rc;
+    if ((rc = st.map(p, sv, true, false, &dp))) return rc;
+    if ((rc = st.map(rho, sv, true, false, &drho))) return rc;
+    if ((rc = st.map_rows(Y, ns, n, ldY, sizeof(double), true, false, &dY))) return rc;
+    if ((rc = st.map_rows(R, ns, n, ldR, sizeof(double), false, true, &dR))) return rc;
+    if (diag && (rc = st.map_rows(diag, ns - 1, n, ldD, esz, false, true, &dD))) return rc;
+    ohcd::KimParams P{n,   (const double *)dT, (const double *)drho, (const double *)dY, ldY,
+                      (double *)dR, ldR, dD, ldD, c->opts.jacobian};
+    HIPCHK(c, ohcd::wave_kim(P, c->wm, diag_fp32, c->stream));
+    return st.finish();
+}
+
 static int eval_common(ohc_ctx *c, int64_t n, const double *y, int64_t ldy, const double *p,
                        double *f, int64_t ldf, double *J, int64_t ldJ) {
     int rc = check_ctx(c);
@@ -709,6 +803,11 @@ static int eval_common(ohc_ctx *c, int64_t n, const double *y, int64_t ldy, cons
     if (n == 0) return OHC_OK;
     if (!c->lane && !c->wave) return fail(c, OHC_ERR_UNSUPPORTED, "no device kernels for this mechanism");
     if (!c->lane && n > 0x7fffffffll) return fail(c, OHC_ERR_ARG, "n too large");
+    int64_t m = 0;
+    if (host_slices(c, n, sizeof(double) * (1 + 2 * (size_t)N + (J ? (size_t)N * N : 0)), &m))
+        return run_slices(c, n, m, [&](int64_t s, int64_t k) {
+            return eval_common(c, k, y + s, ldy, p + s, f + s, ldf, off(J, s), ldJ);
+        });
     Staging st(c);
     void *dy, *dp, *df, *dJ = nullptr;
     if ((rc = st.map_rows(y, N, n, ldy, sizeof(double), true, false, &dy))) return rc;
@@ -744,6 +843,11 @@ static int t_from_f(ohc_ctx *c, int64_t n, const double *h, const double *Y, int
     if (!h || !Y || !T) return fail(c, OHC_ERR_ARG, "null array argument");
     if (n == 0) return OHC_OK;
     if (!c->lane && !c->wave) return fail(c, OHC_ERR_UNSUPPORTED, "no device kernels for this mechanism");
+    int64_t m = 0;
+    if (host_slices(c, n, sizeof(double) * (2 + (size_t)ns) + sizeof(int32_t), &m))
+        return run_slices(c, n, m, [&](int64_t s, int64_t k) {
+            return t_from_f(c, k, h + s, Y + s, ldY, T + s, off(iflag, s), energy);
+        });
     Staging st(c);
     void *dh, *dY, *dT, *dI = nullptr;
     const size_t sv = sizeof(double) * (size_t)n;
@@ -785,6 +889,13 @@ static int mix_common(ohc_ctx *c, int64_t n, const double *T, const double *p, c
         return fail(c, OHC_ERR_UNSUPPORTED, "no device property kernels for this mechanism");
     if (!c->lane && !c->wave)
         return fail(c, OHC_ERR_UNSUPPORTED, "no device property kernels for this mechanism");
+    int64_t m = 0;
+    if (host_slices(c, n, sizeof(double) * (8 + 3 * (size_t)ns), &m))
+        return run_slices(c, n, m, [&](int64_t s, int64_t k) {
+            return mix_common(c, k, T + s, p + s, Y + s, ldY, off(mu, s), off(kappa, s),
+                              off(Dk, s), ldD, off(cp, s), off(ha, s), off(hk, s), off(cv, s),
+                              off(ea, s));
+        });
     Staging st(c);
     const size_t sv = sizeof(double) * (size_t)n;
     void *dT, *dp, *dY, *dmu, *dk, *dD, *dcp, *dha, *dhk, *dcv, *dea;

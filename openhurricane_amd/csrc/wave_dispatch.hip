@@ -11,6 +11,7 @@ namespace ohcd {
     hipError_t wave_integrate_##n(const WaveParams &, int, hipStream_t);             \
     hipError_t wave_eval_##n(const EvalParams &, const WMech &, int, hipStream_t);  \
     hipError_t wave_source_##n(const SourceParams &, const WMech &, hipStream_t);    \
+    hipError_t wave_kim_##n(const KimParams &, const WMech &, int, hipStream_t);     \
     const void *wave_fn_##n();
 OHC_WAVE_DECL(12)
 OHC_WAVE_DECL(16)
@@ -28,10 +29,12 @@ struct Entry {
     hipError_t (*integrate)(const WaveParams &, int, hipStream_t);
     hipError_t (*eval)(const EvalParams &, const WMech &, int, hipStream_t);
     hipError_t (*source)(const SourceParams &, const WMech &, hipStream_t);
+    hipError_t (*kim)(const KimParams &, const WMech &, int, hipStream_t);
     const void *(*fn)();
 };
 
-#define OHC_WAVE(n) Entry{n, &wave_integrate_##n, &wave_eval_##n, &wave_source_##n, &wave_fn_##n}
+#define OHC_WAVE(n) \
+    Entry{n, &wave_integrate_##n, &wave_eval_##n, &wave_source_##n, &wave_kim_##n, &wave_fn_##n}
 const Entry kWave[] = {OHC_WAVE(12), OHC_WAVE(16), OHC_WAVE(24), OHC_WAVE(33),
                        OHC_WAVE(40), OHC_WAVE(48), OHC_WAVE(55), OHC_WAVE(64)};
 #undef OHC_WAVE
@@ -61,6 +64,11 @@ hipError_t wave_eval(const EvalParams &p, const WMech &m, int exact, hipStream_t
 hipError_t wave_source(const SourceParams &p, const WMech &m, hipStream_t s) {
     const Entry *e = pick(m.ns + 1);
     return e ? e->source(p, m, s) : hipErrorInvalidValue;
+}
+
+hipError_t wave_kim(const KimParams &p, const WMech &m, int fp32, hipStream_t s) {
+    const Entry *e = pick(m.ns + 1);
+    return e ? e->kim(p, m, fp32, s) : hipErrorInvalidValue;
 }
 
 // ---- mixture thermo + transport for run-time tables ----------------------

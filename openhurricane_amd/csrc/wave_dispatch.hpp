@@ -9,12 +9,14 @@ namespace ohcd {
 
 struct WMech;
 struct WaveParams;
+struct KimParams;
 
 // Padded ODE size the generic kernels use for n = ns + 1 unknowns (-1: too large).
 int wave_size(int n);
 hipError_t wave_integrate(const WaveParams &p, int grid, hipStream_t s);
 hipError_t wave_eval(const EvalParams &p, const WMech &m, int exact, hipStream_t s);
 hipError_t wave_source(const SourceParams &p, const WMech &m, hipStream_t s);
+hipError_t wave_kim(const KimParams &p, const WMech &m, int fp32, hipStream_t s);
 const void *wave_integrate_fn(int n);
 // mixture thermo / transport and T from h for run-time tables
 hipError_t wave_props(const PropParams &p, const WMech &m, hipStream_t s);

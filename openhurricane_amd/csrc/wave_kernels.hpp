@@ -461,6 +461,10 @@ struct WaveLDS {
     double w[N];      // net production rates omega_k
     double gam[64];   // third-body / fall-off coefficient of each reaction lane of a chunk
     double dinv[N];   // 1 / pivot of each elimination step
+    // pivot row of the current elimination step (element j at [2 j], b at
+    // [2 N]): the stride keeps the writes single 64-bit stores straight from
+    // the row's registers instead of packed 128-bit stores that need copies
+    double prow[2 * N + 1];
     int perm[N];      // pivot lane of each elimination step
 };
 
@@ -586,6 +590,12 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
 // Solve (diag I - h J) x = b with the Jacobian rows in LDS: lane i owns row i;
 // partial pivoting as SquareMatrix::LUDecomposeDPivoting (largest |pivot|),
 // forward substitution folded into the elimination.  Returns x_i in lane i.
+// The elimination steps are unrolled at compile time, so step k touches only
+// the N - k - 1 columns right of the pivot.  The pivot row reaches the other
+// lanes through LDS (OHC_LU_LANE_READS = 1: through v_readlane pairs).
+#ifndef OHC_LU_LANE_READS
+#define OHC_LU_LANE_READS 0
+#endif
 template <int N>
 __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, double b,
                                                 double diag = 1.0) {
@@ -603,8 +613,8 @@ __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, 
         a[j] = (act && j == lane) ? v + diag : v;
     }
     int step = act ? -1 : N;  // elimination step at which this row became the pivot
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
+    sfor<0, N>([&](auto kc) {
+        constexpr int k = kc;
         if (k < n) {
             const double v = step < 0 ? fabs(a[k]) : -1.0;
             const double m = wmax(v);
@@ -616,30 +626,37 @@ __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, 
                 L.perm[k] = p;
                 L.dinv[k] = rp;
             }
+            const bool el = step < 0 && lane != p;
             if (lane == p) step = k;
-            const bool el = step < 0;
             const double l = el ? a[k] * rp : 0.0;
             if (el) a[k] = l;
             // columns j >= n hold zeros in every row and stay zero, so the
-            // update needs no per-column predicate.  (Publishing the pivot row
-            // through LDS instead of lane reads measured 7 % slower, and
-            // ds_bpermute broadcasts 3 % slower: at this register budget the
-            // broadcast loads serialise on their waits.)
-#pragma unroll
-            for (int j = k + 1; j < N; ++j) a[j] = fma(-l, rdlane(a[j], p), a[j]);
+            // update needs no per-column predicate
+#if OHC_LU_LANE_READS
+            sfor<k + 1, N>([&](auto j) { a[j] = fma(-l, rdlane(a[j], p), a[j]); });
             b = fma(-l, rdlane(b, p), b);
+#else
+            if (lane == p) {
+                sfor<k + 1, N>([&](auto j) { L.prow[2 * j] = a[j]; });
+                L.prow[2 * N] = b;
+            }
+            __syncthreads();
+            sfor<k + 1, N>([&](auto j) { a[j] = fma(-l, L.prow[2 * j], a[j]); });
+            b = fma(-l, L.prow[2 * N], b);
+            // (the block is one wavefront, whose LDS accesses complete in
+            // program order: the next step's row cannot overwrite these reads)
+#endif
         }
-    }
+    });
     double x = 0.0;
-#pragma unroll
-    for (int kk = 0; kk < N; ++kk) {
-        const int k = N - 1 - kk;
+    sfor<0, N>([&](auto kk) {
+        constexpr int k = N - 1 - kk;
         if (k < n) {
             const double xk = rdlane(b, L.perm[k]) * L.dinv[k];
             if (step < k) b = fma(-a[k], xk, b);
             if (lane == k) x = xk;
         }
-    }
+    });
     return x;
 }
 
@@ -958,6 +975,152 @@ __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const Wa
     }
 }
 
+// ---- MTS explicit Euler (the reference GPU path's MTSEuler solver type:
+// solveChemODEsMTS, CUDAChemistrySourceODEs.cu:684-778, with timescale /
+// MTS_whichGroup / MTS_GroupDT :199-309).  The species are grouped by the
+// decade of their time scale tci_k = min(200, 1 / (|d omega_k/d c_k| + 1e-20))
+// relative to the flow step; group by group, fastest first, explicit Euler
+// steps of 0.8 times the group's smallest time scale advance the whole state
+// with the faster groups' rates frozen, until the group's increments are
+// within atol + rtol |y| (the last group runs to the end of the step).  A
+// group without species is skipped, the time-scale sums start at zero and a
+// cell fails after max_steps steps (where the reference's device code leaves
+// these undefined).
+
+// d(omega_k)/d(c_k) of the mass-action terms, kf and kr held fixed
+// (reactionTable::dWdciAllSpecies, reactionTableCUDA.cu:331-385), lane k.
+template <int N>
+__device__ __forceinline__ double wave_mts_diag(const WMech &M, WaveLDS<N> &L, double yv,
+                                                int exact) {
+    const int lane = threadIdx.x, ns = M.ns;
+    const double T = rdlane(yv, ns);
+    const TFac tf = tfac(T);
+    if (lane < ns) {
+        L.c[lane] = fmax(yv, 0.0);
+        double G, dG, Cp, H, dCp;
+        wthermo(M, lane, tf, G, dG, Cp, H, dCp);
+        L.G[lane] = G;
+        L.dG[lane] = dG;
+    }
+    if (lane < N) L.w[lane] = 0.0;
+    __syncthreads();
+    for (int base = 0; base < M.nr; base += 64) {
+        const int r = base + lane;
+        if (r < M.nr) {
+            WRxn o;
+            wreaction<true>(M, r, tf, L.c, L.G, L.dG, exact, o);
+#pragma unroll
+            for (int i = 0; i < kWP; ++i) {
+                if (i < o.nf) {
+                    const int a = o.fi[i];
+                    double d = -o.fnu[i] * o.dqf[i];
+#pragma unroll
+                    for (int j = 0; j < kWP; ++j)
+                        if (j < o.nb && o.bi[j] == a) d += o.bnu[j] * o.dqf[i];
+                    lds_add(&L.w[a], d);
+                }
+                if (i < o.nb) {
+                    const int a = o.bi[i];
+                    double d = -o.bnu[i] * o.dqr[i];
+#pragma unroll
+                    for (int j = 0; j < kWP; ++j)
+                        if (j < o.nf && o.fi[j] == a) d += o.fnu[j] * o.dqr[i];
+                    lds_add(&L.w[a], d);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const double d = lane < ns ? L.w[lane] : 0.0;
+    __syncthreads();
+    return d;
+}
+
+__device__ __forceinline__ double wmin(double v) { return -wmax(-v); }
+
+template <int N, int FLOW>
+__global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void mts_wave_kernel(const WaveParams WP) {
+    const LaneParams &P = WP.L;
+    const WMech &M = WP.M;
+    const int lane = threadIdx.x, ns = M.ns, n = ns + 1;
+    __shared__ WaveLDS<N> L;
+    __shared__ unsigned long long s_cell;
+    const bool sp = lane < ns;
+    const double W = sp ? M.sp[SP_W * 64 + lane] : 1.0;
+    constexpr double kNone = 1.0e300;
+    long long c_cells = 0, c_steps = 0, c_maxs = 0, c_fail = 0;
+    double dtmin = __builtin_huge_val();
+    for (;;) {
+        if (lane == 0) s_cell = atomicAdd(P.counter, 1ull);
+        __syncthreads();
+        const unsigned long long qi = s_cell;
+        __syncthreads();
+        if (qi >= (unsigned long long)P.n) break;
+        const long long cell = P.order ? (long long)P.order[qi] : (long long)qi;
+        const double T0 = P.T[cell];
+        const double tn = P.dt[cell] * P.dtFactor;
+        const double rho = P.rho[cell];
+        const double c0 = sp ? rho * P.Y[lane * P.ldY + cell] / W : 0.0;
+        double y = sp ? c0 : (lane == ns ? T0 : 0.0);
+        // timescale / MTS_whichGroup / MTS_GroupDT
+        const double dd = wave_mts_diag<N>(M, L, y, WP.exact_dkp);
+        const double tci = sp ? fmin(200.0, 1.0 / (fabs(dd) + 1e-20)) : kNone;
+        const double tmin = wmin(tci);
+        const int Nm = tn > tmin ? (int)fmax(0.0, floor(log10(tn / tmin))) + 1 : 1;
+        const int ig = sp ? Nm - mts_group(tn, tci) : Nm;  // group index, fastest = 0
+        double dtDid = 0.0;
+        long long nst = 0;
+        bool failed = false;
+        int im = (int)wmin(sp ? (double)ig : kNone);
+        while (im < Nm && !failed) {
+            const double dtG = wmin(sp && ig == im ? tci : kNone);
+            while (dtDid < tn) {
+                if (nst >= P.max_steps) {
+                    failed = true;
+                    break;
+                }
+                double dt = 0.8 * dtG;
+                if (dtDid + dt > tn) dt = tn - dtDid;
+                double f = wave_eval<N, FLOW, false>(M, L, y, WP.exact_dkp);
+                if (sp && ig < im) f = 0.0;  // faster groups are frozen
+                dtDid += dt;
+                const double err = dt * f, ay0 = fabs(y);
+                if (lane < n) y += err;
+                const double tol = P.atol + P.rtol * fmax(fabs(y), ay0);
+                const double e = wmax(sp && ig == im ? fabs(err) / tol : 0.0);
+                ++nst;
+                if (im != Nm - 1 && e < 1.0) break;
+            }
+            im = (int)fmin((double)Nm, wmin(sp && ig > im ? (double)ig : kNone));
+        }
+        if (lane == 0 && P.cost) P.cost[cell] = (int)(nst < 0x7fffffffll ? nst : 0x7fffffffll);
+        if (P.mode == 0) {
+            if (sp) P.R[lane * P.ldR + cell] = (y - c0) * (1.0 / tn) * W;
+        } else {
+            // device ciToYi (CUDAChemistrySourceODEs.cu:545-586): Y = c W / sum c W
+            const double r = wsum(sp ? y * W : 0.0);
+            if (sp) P.Yout[lane * P.ldY + cell] = y * W * (1.0 / r);
+            const double Tn = rdlane(y, ns);
+            if (lane == 0) P.Tout[cell] = Tn;
+        }
+        ++c_cells;
+        c_steps += nst;
+        if (failed) ++c_fail;
+        c_maxs = nst > c_maxs ? nst : c_maxs;
+        dtmin = fmin(dtmin, P.subDt[cell]);
+    }
+    if (lane == 0) {
+        atomicAdd(&P.stats[0], (unsigned long long)c_cells);
+        atomicAdd(&P.stats[1], (unsigned long long)c_steps);
+        atomicAdd(&P.stats[2], (unsigned long long)c_steps);
+        atomicAdd(&P.stats[4], (unsigned long long)c_steps);
+        atomicMax(&P.stats[5], (unsigned long long)c_maxs);
+        if (dtmin < __builtin_huge_val() && dtmin >= 0.0)
+            atomicMin(&P.stats[6], (unsigned long long)__double_as_longlong(dtmin));
+        atomicAdd(&P.stats[7], (unsigned long long)c_fail);
+    }
+}
+
 // Source terms and their (coupled) diagonal, one wavefront per cell; same
 // definitions as source_lane_kernel.
 template <int N>
@@ -988,6 +1151,103 @@ __global__ __launch_bounds__(64) void source_wave_kernel(const SourceParams P, c
         if (lane < l)
             P.D[lane * P.ldD + cell] = row[lane] - row[l] * (W / Wl) + row[ns] * W * (el - ea) * b;
     }
+}
+
+// The GPU path's calculateSourceTermsImp[Async][Hybrid] (calcChemSource-
+// TermsKimDiagonal, CUDAChemSourceKimDiagonalKernel.cu:39-105, with
+// reactionTable::qfr<dOmegaType>, reactionTableCUDA.inl:142-273), one
+// wavefront per cell: R_k = omega_k W_k (FP64) and the mass-action diagonal
+// d(omega_k)/d(c_k) of species k < ns-1 with the last species substituted
+// (d c_ns / d c_k = -W_k / W_ns), accumulated and stored as DT (double, or
+// float for the Hybrid variant: each term formed in FP64, rounded to float,
+// summed in float).
+struct KimParams {
+    int64_t n;
+    const double *T, *rho, *Y;
+    int64_t ldY;
+    double *R;
+    int64_t ldR;
+    void *D;  // DT [ns-1][ldD], nullable
+    int64_t ldD;
+    int exact;
+};
+
+template <class DT>
+__device__ __forceinline__ void lds_add_t(DT *p, DT v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <int N, class DT>
+__global__ __launch_bounds__(64) void kim_wave_kernel(const KimParams P, const WMech M) {
+    __shared__ WaveLDS<N> L;
+    __shared__ DT s_d[64];
+    const int lane = threadIdx.x, ns = M.ns;
+    const int64_t cell = blockIdx.x;
+    if (cell >= P.n) return;
+    const bool sp = lane < ns;
+    const double T = P.T[cell], rho = P.rho[cell];
+    const double W = sp ? M.sp[SP_W * 64 + lane] : 1.0;
+    const TFac tf = tfac(T);
+    if (sp) {
+        L.c[lane] = fmax(rho * P.Y[lane * P.ldY + cell] / W, 0.0);
+        double G, dG, Cp, H, dCp;
+        wthermo(M, lane, tf, G, dG, Cp, H, dCp);
+        L.G[lane] = G;
+        L.dG[lane] = dG;
+    }
+    if (lane < N) L.w[lane] = 0.0;
+    s_d[lane] = DT(0);
+    __syncthreads();
+    const int last = ns - 1;
+    const double Wns = M.sp[SP_W * 64 + last];
+    for (int base = 0; base < M.nr; base += 64) {
+        const int r = base + lane;
+        if (r < M.nr) {
+            WRxn o;
+            wreaction<true>(M, r, tf, L.c, L.G, L.dG, P.exact, o);
+#pragma unroll
+            for (int i = 0; i < kWP; ++i) {
+                if (i < o.nf) lds_add(&L.w[o.fi[i]], -o.fnu[i] * o.q);
+                if (i < o.nb) lds_add(&L.w[o.bi[i]], o.bnu[i] * o.q);
+            }
+            // qfr<dOmegaType>: forward participants (side 0, derivative dqf)
+            // then reverse participants (side 1, dqr)
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const int np = side == 0 ? o.nf : o.nb, no = side == 0 ? o.nb : o.nf;
+                const int *id = side == 0 ? o.fi : o.bi;
+                const int *oid = side == 0 ? o.bi : o.fi;
+                const double *nu = side == 0 ? o.fnu : o.bnu;
+                const double *onu = side == 0 ? o.bnu : o.fnu;
+                const double *dq = side == 0 ? o.dqf : o.dqr;
+#pragma unroll
+                for (int i = 0; i < kWP; ++i) {
+                    if (i >= np) continue;
+                    const int spi = id[i];
+                    if (spi != last) {
+                        double d = -nu[i] * dq[i];
+#pragma unroll
+                        for (int j = 0; j < kWP; ++j)
+                            if (j < no && oid[j] == spi) d += onu[j] * dq[i];
+                        lds_add_t<DT>(&s_d[spi], (DT)d);
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < kWP; ++j) {
+                            if (j < np && id[j] != spi)
+                                lds_add_t<DT>(&s_d[id[j]],
+                                              (DT)(M.sp[SP_W * 64 + id[j]] / Wns * nu[j] * dq[i]));
+                            if (j < no && oid[j] != spi)
+                                lds_add_t<DT>(&s_d[oid[j]],
+                                              (DT)(-M.sp[SP_W * 64 + oid[j]] / Wns * onu[j] * dq[i]));
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (sp) P.R[lane * P.ldR + cell] = L.w[lane] * W;
+    if (P.D && lane < last) ((DT *)P.D)[lane * P.ldD + cell] = s_d[lane];
 }
 
 // dy/dt and J at given states, one wavefront per cell.

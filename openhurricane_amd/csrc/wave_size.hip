@@ -13,7 +13,10 @@ namespace ohcd {
 
 hipError_t OHC_CAT(wave_integrate_, OHC_WAVE_N)(const WaveParams &p, int grid, hipStream_t s) {
     constexpr int N = OHC_WAVE_N;
-    if (p.L.integrator == 3) {
+    if (p.L.integrator == 4) {
+        if (p.L.flow == 0) hipLaunchKernelGGL((mts_wave_kernel<N, 0>), dim3(grid), dim3(64), 0, s, p);
+        else hipLaunchKernelGGL((mts_wave_kernel<N, 1>), dim3(grid), dim3(64), 0, s, p);
+    } else if (p.L.integrator == 3) {
         if (p.L.flow == 0) hipLaunchKernelGGL((ie_wave_kernel<N, 0, 3>), dim3(grid), dim3(64), 0, s, p);
         else hipLaunchKernelGGL((ie_wave_kernel<N, 1, 3>), dim3(grid), dim3(64), 0, s, p);
     } else if (p.L.integrator == 2) {
@@ -39,6 +42,14 @@ hipError_t OHC_CAT(wave_eval_, OHC_WAVE_N)(const EvalParams &p, const WMech &m, 
 hipError_t OHC_CAT(wave_source_, OHC_WAVE_N)(const SourceParams &p, const WMech &m,
                                               hipStream_t s) {
     hipLaunchKernelGGL((source_wave_kernel<OHC_WAVE_N>), dim3((unsigned)p.n), dim3(64), 0, s, p, m);
+    return hipGetLastError();
+}
+
+hipError_t OHC_CAT(wave_kim_, OHC_WAVE_N)(const KimParams &p, const WMech &m, int fp32,
+                                           hipStream_t s) {
+    const dim3 g((unsigned)p.n);
+    if (fp32) hipLaunchKernelGGL((kim_wave_kernel<OHC_WAVE_N, float>), g, dim3(64), 0, s, p, m);
+    else hipLaunchKernelGGL((kim_wave_kernel<OHC_WAVE_N, double>), g, dim3(64), 0, s, p, m);
     return hipGetLastError();
 }
 

@@ -1417,6 +1417,144 @@ static int ee_solve(ie_ws *s, double *tn, double *dt0, double *y) {
     return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* Reference GPU integrator, multi-time-scale variant (MTSEuler solver    */
+/* type, CS/src/CUDAChemistrySource.cpp:104-105, :389-392):               */
+/* solveChemODEsMTS (CS/src/CUDAChemistrySourceODEs.cu:684-778) with      */
+/* timescale / MTS_whichGroup / MTS_GroupDT (:199-309) and                */
+/* reactionTable::dWdciAllSpecies (RX/cuda/src/reactionTableCUDA.cu:      */
+/* 331-385).  Where the reference's device code is undefined, the evident */
+/* intent is restated: the time-scale sums start at zero (the reference   */
+/* accumulates into uninitialised shared memory), a group that holds no   */
+/* species is skipped (the reference steps with dt = 0.8 * -1), and a     */
+/* cell fails after max_steps explicit steps (the reference has no cap).  */
+/* Concentrations enter the rates clamped at 0, as in every other path.   */
+/* ------------------------------------------------------------------ */
+
+/* d(omega_k)/d(c_k) of the mass-action terms with kf, kr (third-body and
+ * fall-off factors included) held fixed: dWdciAllSpecies */
+static void mts_diag(const orc_mech *m, double T, const double *c, double *d) {
+    const int P = OHC_MAXP, ns = m->ns;
+    thermo_ws w;
+    species_thermo(m, T, &w);
+    for (int k = 0; k < ns; ++k) d[k] = 0.0;
+    for (int r = 0; r < m->nr; ++r) {
+        const double kf = rate_k(m, r, m->arr + 3 * r, T, c);
+        const double kr = rxn_kr(m, r, kf, T, c, &w);
+        for (int side = 0; side < 2; ++side) {
+            const int np = side == 0 ? m->nf[r] : m->nb[r];
+            const int *id = side == 0 ? m->fi + r * P : m->bi + r * P;
+            const double *nu = side == 0 ? m->fnu + r * P : m->bnu + r * P;
+            const double *od = side == 0 ? m->ford + r * P : m->bord + r * P;
+            const int no = side == 0 ? m->nb[r] : m->nf[r];
+            const int *oid = side == 0 ? m->bi + r * P : m->fi + r * P;
+            const double *onu = side == 0 ? m->bnu + r * P : m->fnu + r * P;
+            for (int i = 0; i < np; ++i) {
+                double pf = side == 0 ? kf : kr;
+                for (int k = 0; k < np; ++k) {
+                    const double ck = c[id[k]], ok = od[k];
+                    if (i == k) {
+                        if (ok < 1.0) pf = ck > TINY ? pf * ok * pow(ck + VERYTINY, ok - 1.0) : 0.0;
+                        else if (ok != 1.0) pf *= ok * pow(ck, ok - 1.0);
+                    } else {
+                        pf *= pow(ck, ok);
+                    }
+                }
+                d[id[i]] -= nu[i] * pf;
+                for (int j = 0; j < no; ++j)
+                    if (oid[j] == id[i]) d[id[i]] += onu[j] * pf;
+            }
+        }
+    }
+}
+
+/* timescale: tci_k = min(200, 1 / (|d omega_k / d c_k| + 1e-20)) */
+void orc_mts_timescales(const orc_mech *m, double T, const double *c, double *tci) {
+    double cc[NMAX], d[NMAX];
+    for (int k = 0; k < m->ns; ++k) cc[k] = dmax(c[k], 0.0);
+    mts_diag(m, T, cc, d);
+    for (int k = 0; k < m->ns; ++k) tci[k] = dmin(200.0, 1.0 / (fabs(d[k]) + 1e-20));
+}
+
+/* MTS_whichGroup: 1 + decades of dtBase / tci (1 for tci >= dtBase) */
+static int mts_group(double dtBase, double tci) {
+    if (!(dtBase > tci)) return 1;
+    const int g = (int)log10(dtBase / tci);
+    return (g > 0 ? g : 0) + 1;
+}
+
+static int cmp_d(const void *a, const void *b) {
+    const double x = *(const double *)a, y = *(const double *)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* One cell over tn: y = [c_1..c_ns, T] advanced in place; *Nm_out receives
+ * the number of groups; returns 0 or OHC_ERR_NUMERIC (step cap). */
+static int mts_cell(ie_ws *s, double tn, double *y, int *Nm_out) {
+    const orc_mech *m = s->m;
+    const int ns = m->ns, n = ns + 1;
+    double tci[NMAX], srt[NMAX], dtG[NMAX], f[NMAX];
+    int grp[NMAX];
+    orc_mts_timescales(m, y[ns], y, tci);
+    for (int k = 0; k < ns; ++k) grp[k] = mts_group(tn, tci[k]);
+    /* MTS_GroupDT: the smallest time scale of each group, fastest group first */
+    memcpy(srt, tci, sizeof(double) * ns);
+    qsort(srt, (size_t)ns, sizeof(double), cmp_d);
+    const int Nm = tn > srt[0] ? (int)dmax(0.0, floor(log10(tn / srt[0]))) + 1 : 1;
+    for (int i = 0; i < Nm; ++i) {
+        dtG[i] = -1.0;
+        for (int j = 0; j < ns; ++j)
+            if (Nm - mts_group(tn, srt[j]) == i) {
+                dtG[i] = srt[j];
+                break;
+            }
+    }
+    if (Nm_out) *Nm_out = Nm;
+    double dtDid = 0.0;
+    int64_t nsteps = 0;
+    for (int im = 0; im < Nm; ++im) {
+        if (!(dtG[im] > 0.0)) continue;
+        while (dtDid < tn) {
+            if (nsteps >= s->o->max_steps) return OHC_ERR_NUMERIC;
+            double dt = 0.8 * dtG[im];
+            if (dtDid + dt > tn) dt = tn - dtDid;
+            orc_dydt(m, s->o->flow_type, s->pr, y, f);
+            s->rhs++;
+            for (int k = 0; k < ns; ++k)
+                if (Nm - grp[k] < im) f[k] = 0.0; /* faster groups are frozen */
+            dtDid += dt;
+            double emax = 0.0;
+            for (int i = 0; i < n; ++i) {
+                const double err = dt * f[i], ay0 = fabs(y[i]);
+                y[i] += err;
+                const double tol = s->o->atol + s->o->rtol * dmax(fabs(y[i]), ay0);
+                /* fmax: a NaN increment (a cell the explicit steps blew up) does not
+                 * decide the group's end, as in the device reductions */
+                if (i < ns && Nm - grp[i] == im) emax = fmax(emax, fabs(err) / tol);
+            }
+            ++nsteps;
+            s->steps++;
+            s->attempts++;
+            s->cell_steps++;
+            if (im != Nm - 1 && emax < 1.0) break;
+        }
+    }
+    return 0;
+}
+
+int orc_mts_cell(const orc_mech *m, const ohc_opts *o, double pr, double tn, double *y,
+                 int *Nm, int64_t *steps) {
+    ie_ws *s = (ie_ws *)calloc(1, sizeof(ie_ws));
+    s->m = m;
+    s->o = o;
+    s->n = m->ns + 1;
+    s->pr = pr;
+    const int rc = mts_cell(s, tn, y, Nm);
+    if (steps) *steps = s->steps;
+    free(s);
+    return rc;
+}
+
 typedef struct {
     const orc_mech *m;
     const ohc_opts *o;
@@ -1448,8 +1586,9 @@ static void *solve_range(void *arg) {
         const double dtc = j->dt[cell] * j->dtFactor;
         double dtI = j->subDt[cell] > 0 ? j->subDt[cell] : dtc; /* chemistrySource.inl:86-95 */
         if (j->cell_steps) j->cell_steps[cell] = 0;
-        if (j->o->integrator == 1) {
-            /* CUDAChemistrySource::solve / solveUpdate: every cell, dt0 = subDt */
+        if (j->o->integrator == 1 || j->o->integrator == 4) {
+            /* CUDAChemistrySource::solve / solveUpdate: every cell, dt0 = subDt
+             * (explicit Euler) or the MTS variant, which leaves subDt as it is */
             const double pi = j->p[cell], rhoi = j->rho[cell];
             const double *Ysrc = j->update ? j->Y : j->Yin;
             double y[NMAX];
@@ -1462,10 +1601,15 @@ static void *solve_range(void *arg) {
             s->cell_steps = 0;
             double left = dtc, sub = dtI;
             int rc = 0;
-            while (left > 0.0 && !rc) {
-                double dtn = left;
-                rc = ee_solve(s, &dtn, &sub, y);
-                left -= dtn;
+            if (j->o->integrator == 4) {
+                sub = j->subDt[cell];
+                rc = mts_cell(s, dtc, y, NULL);
+            } else {
+                while (left > 0.0 && !rc) {
+                    double dtn = left;
+                    rc = ee_solve(s, &dtn, &sub, y);
+                    left -= dtn;
+                }
             }
             if (rc) {
                 j->rc = rc;
@@ -1760,6 +1904,94 @@ void orc_chem_source(const orc_mech *m, const ohc_opts *o, long n, const double 
         for (int k = 0; k < ns; ++k) R[k * ldR + cell] = w[k] * m->W[k];
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* The GPU path's calculateSourceTermsImp[Async][Hybrid]                 */
+/* (CS/src/CUDAChemistrySource.cpp:152-250): calcChemSourceTermsKim-     */
+/* Diagonal (CS/src/CUDAChemSourceKimDiagonalKernel.cu:39-105) with      */
+/* reactionTable::omegaCoupled3 / qfr<dOmegaType> (RX/cuda/include/      */
+/* reactionTableCUDA.inl:142-300).  R_k = omega_k W_k in FP64; the        */
+/* diagonal d(omega_k)/d(c_k) of the mass-action terms (kf, kr with their */
+/* third-body and fall-off factors held fixed) for k < ns-1, with the last */
+/* species eliminated by mass conservation (d c_ns / d c_k = -W_k/W_ns),  */
+/* accumulated in dOmegaType: double, or float for the Hybrid variant    */
+/* (every product of the diagonal in float, as the reference casts).     */
+/* ------------------------------------------------------------------ */
+#define KIM_DIAG(TYPE, POW)                                                                   \
+    do {                                                                                      \
+        for (int side = 0; side < 2; ++side) {                                                \
+            const int np = side == 0 ? m->nf[r] : m->nb[r];                                   \
+            const int *id = side == 0 ? m->fi + r * P : m->bi + r * P;                        \
+            const double *nu = side == 0 ? m->fnu + r * P : m->bnu + r * P;                   \
+            const double *od = side == 0 ? m->ford + r * P : m->bord + r * P;                 \
+            const int no = side == 0 ? m->nb[r] : m->nf[r];                                   \
+            const int *oid = side == 0 ? m->bi + r * P : m->fi + r * P;                       \
+            const double *onu = side == 0 ? m->bnu + r * P : m->fnu + r * P;                  \
+            for (int i = 0; i < np; ++i) {                                                    \
+                TYPE pf = (TYPE)(side == 0 ? kf : kr);                                        \
+                for (int k = 0; k < np; ++k) {                                                \
+                    const TYPE ck = (TYPE)c[id[k]], ok = (TYPE)od[k];                         \
+                    if (i == k) {                                                             \
+                        if (ok < (TYPE)1) pf = ck > TINY ? pf * ok * POW(ck, ok - (TYPE)1) : (TYPE)0; \
+                        else if (ok != (TYPE)1) pf *= ok * POW(ck, ok - (TYPE)1);             \
+                    } else {                                                                  \
+                        pf *= POW(ck, ok);                                                    \
+                    }                                                                         \
+                }                                                                             \
+                const int spi = id[i];                                                        \
+                if (spi != ns - 1) {                                                          \
+                    acc[spi] += -(TYPE)nu[i] * pf;                                            \
+                    for (int j = 0; j < no; ++j)                                              \
+                        if (oid[j] == spi) acc[spi] += (TYPE)onu[j] * pf;                     \
+                } else {                                                                      \
+                    const TYPE Wns = (TYPE)m->W[spi];                                         \
+                    for (int j = 0; j < np; ++j)                                              \
+                        if (id[j] != spi) acc[id[j]] += (TYPE)m->W[id[j]] / Wns * (TYPE)nu[j] * pf; \
+                    for (int j = 0; j < no; ++j)                                              \
+                        if (oid[j] != spi) acc[oid[j]] += -(TYPE)m->W[oid[j]] / Wns * (TYPE)onu[j] * pf; \
+                }                                                                             \
+            }                                                                                 \
+        }                                                                                     \
+    } while (0)
+
+void orc_chem_source_kim(const orc_mech *m, long n, const double *T, const double *p,
+                         const double *rho, const double *Y, long ldY, double *R, long ldR,
+                         void *diag, long ldD, int fp32) {
+    const int ns = m->ns, P = OHC_MAXP;
+    for (long cell = 0; cell < n; ++cell) {
+        const double Ti = T[cell], pi = p[cell], rhoi = rho[cell];
+        double c[NMAX], w[NMAX], dd[NMAX];
+        float df[NMAX];
+        for (int k = 0; k < ns; ++k) {
+            c[k] = dmax(rhoi * Y[k * ldY + cell] / m->W[k], 0.0);
+            w[k] = dd[k] = 0.0;
+            df[k] = 0.0f;
+        }
+        thermo_ws tw;
+        species_thermo(m, Ti, &tw);
+        for (int r = 0; r < m->nr; ++r) {
+            double qf, qr, kf, kr;
+            const double q = rxn_q(m, r, pi, Ti, c, &tw, &qf, &qr, &kf, &kr);
+            for (int i = 0; i < m->nf[r]; ++i) w[m->fi[r * P + i]] -= m->fnu[r * P + i] * q;
+            for (int i = 0; i < m->nb[r]; ++i) w[m->bi[r * P + i]] += m->bnu[r * P + i] * q;
+            if (fp32) {
+                float *acc = df;
+                KIM_DIAG(float, powf);
+            } else {
+                double *acc = dd;
+                KIM_DIAG(double, pow);
+            }
+        }
+        for (int k = 0; k < ns; ++k) R[k * ldR + cell] = w[k] * m->W[k];
+        if (diag) {
+            for (int k = 0; k < ns - 1; ++k) {
+                if (fp32) ((float *)diag)[k * ldD + cell] = df[k];
+                else ((double *)diag)[k * ldD + cell] = dd[k];
+            }
+        }
+    }
+}
+#undef KIM_DIAG
 
 /* ------------------------------------------------------------------ */
 /* T from h: thermoList.cpp:504-533, limit :152-163                     */

@@ -13,6 +13,11 @@
  *   omega / RHS   mass and element conservation, dh/dt = 0 at constant p
  *   Jacobian      central finite differences of the RHS
  *   LU            numpy.linalg.solve, pivot-requiring matrices
+ *   Kim diagonal  J_kk - (W_k / W_ns) J_k,ns of the analytic Jacobian; the
+ *                 FP32 variant within float rounding of the FP64 one
+ *   MTS           time scales = 1 / |Jacobian diagonal| (mass-action part);
+ *                 closed form of an isomerisation A -> B; one-group cells =
+ *                 forward Euler, error O(tn^2) against Radau
  *   integrator    scipy Radau at rtol 1e-12 (accuracy), conservation; the
  *                 trapezoidal rule at second order; BDF23 between step
  *                 tolerances 1e-3 and 1e-4 (and its underflow at 1e-8)
@@ -69,10 +74,24 @@ int orc_chem_solve_update(const orc_mech *m, const ohc_opts *o, long n, double *
                           const double *dt, double dtFactor, double *subDt, int nthreads,
                           ohc_stats *stats, int *cell_steps);
 
+/* MTS explicit Euler (integrator 4): species time scales at (T, c) and one
+ * cell advanced over tn (y = [c, T] in place); *Nm = number of time-scale
+ * groups, *steps = explicit steps.  Returns 0 or OHC_ERR_NUMERIC. */
+void orc_mts_timescales(const orc_mech *m, double T, const double *c, double *tci);
+int orc_mts_cell(const orc_mech *m, const ohc_opts *o, double pr, double tn, double *y,
+                 int *Nm, int64_t *steps);
+
 /* calculateSourceTerms (diagJ NULL) / calculateSourceTermsImp. */
 void orc_chem_source(const orc_mech *m, const ohc_opts *o, long n, const double *T,
                      const double *p, const double *rho, const double *Y, long ldY, double *R,
                      long ldR, double *diagJ, long ldD, int withoutLastSpec);
+
+/* The GPU path's calculateSourceTermsImp (Kim diagonal): R [ns] in FP64 and
+ * the mass-action diagonal with the last species substituted [ns-1], as
+ * double (fp32 = 0) or float (fp32 = 1, the Hybrid variant) into diag. */
+void orc_chem_source_kim(const orc_mech *m, long n, const double *T, const double *p,
+                         const double *rho, const double *Y, long ldY, double *R, long ldR,
+                         void *diag, long ldD, int fp32);
 
 /* thermoList::THa_p over n cells. */
 void orc_T_from_h(const orc_mech *m, const ohc_opts *o, long n, const double *h,

@@ -55,6 +55,11 @@ def lib():
         L.orc_mix_thermo.argtypes = [_V, C.c_long, _V, _V, _V, C.c_long, _V, _V, _V, _V]
         L.orc_mix_props.argtypes = [_V, C.c_long, _V, _V, _V, C.c_long, _V, _V, _V, C.c_long,
                                     _V, _V, _V]
+        L.orc_mts_timescales.argtypes = [_V, C.c_double, _V, _V]
+        L.orc_chem_source_kim.argtypes = [_V, C.c_long, _V, _V, _V, _V, C.c_long, _V, C.c_long,
+                                          _V, C.c_long, C.c_int]
+        L.orc_mts_cell.restype = C.c_int
+        L.orc_mts_cell.argtypes = [_V, _V, C.c_double, C.c_double, _V, _V, _V]
         _L = L
     return _L
 
@@ -115,6 +120,20 @@ class Oracle:
                            _p(J), int(exact_dkp))
         return f, J
 
+    def mts_timescales(self, T, c):
+        tci = np.zeros(self.ns)
+        lib().orc_mts_timescales(self._h, T, _p(np.ascontiguousarray(c, dtype=float)), _p(tci))
+        return tci
+
+    def mts_cell(self, opts, y, pr, tn):
+        """One cell of the MTS explicit integrator: returns (y_new, groups, steps)."""
+        y = np.ascontiguousarray(y, dtype=float).copy()
+        nm, st = C.c_int(), C.c_int64()
+        rc = lib().orc_mts_cell(self._h, C.byref(opts), pr, tn, _p(y), C.byref(nm), C.byref(st))
+        if rc:
+            raise RuntimeError(f"oracle MTS cell failed ({rc})")
+        return y, nm.value, st.value
+
     @staticmethod
     def lu_solve(A, b):
         A = np.ascontiguousarray(A, dtype=float).copy()
@@ -160,6 +179,16 @@ class Oracle:
         lib().orc_chem_source(self._h, C.byref(opts), n, _p(T), _p(p), _p(rho), _p(Y), n, _p(R),
                               n, _p(D), n, int(without_last))
         return (R, D) if diag else R
+
+    def chem_source_kim(self, T, p, rho, Y, fp32=False):
+        """The GPU path's source terms R [ns, n] and Kim diagonal [ns-1, n]
+        (float32 for the Hybrid variant)."""
+        n = T.shape[0]
+        R = np.zeros((self.ns, n))
+        D = np.zeros((self.ns - 1, n), dtype=np.float32 if fp32 else np.float64)
+        lib().orc_chem_source_kim(self._h, n, _p(T), _p(p), _p(rho), _p(Y), n, _p(R), n, _p(D), n,
+                                  int(fp32))
+        return R, D
 
     def T_from_h(self, opts, h, p, Y, T0):
         n = h.shape[0]

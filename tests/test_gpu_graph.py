@@ -5,8 +5,10 @@ ohc_chem_solve_update (no statistics requested, so nothing synchronises) into
 a graph and replay it every flow step.  The captured work is the control-word
 reset kernel and the integrator kernel -- no memset nodes, no host copies.
 Each replay, with the program's own eager work and synchronisations between
-replays, must give the eager result bit for bit.  The longest-first schedule
-is off here: it sorts with a library radix sort, which is kept out of graphs.
+replays, must give the eager result bit for bit.  With the longest-first
+schedule on (the benchmarked configuration) the graph also holds the
+counting-sort kernels that order the next call; the schedule buffers are
+allocated by the eager call before the capture (same n).
 """
 import numpy as np
 import pytest
@@ -17,7 +19,7 @@ pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not gpu_available(), reason="needs a HIP GPU")]
 
 
-def _run(mech, states, kernel, n, replays=3):
+def _run(mech, states, kernel, n, replays=3, schedule=0):
     import torch
 
     from openhurricane_amd import _lib
@@ -25,7 +27,7 @@ def _run(mech, states, kernel, n, replays=3):
 
     dev = torch.device("cuda:0")
     opts = _lib.default_opts(atol=1e-10, rtol=1e-6, atol_inc=1e-7, rtol_inc=1e-3, jacobian=1,
-                             device_pointers=1, schedule=0, kernel=kernel)
+                             device_pointers=1, schedule=schedule, kernel=kernel)
     cs = ChemistrySource(mech, opts=opts)
     g = {k: torch.from_numpy(np.ascontiguousarray(v[..., :n])).to(dev)
          for k, v in states.items() if k != "kind"}
@@ -58,23 +60,25 @@ def _run(mech, states, kernel, n, replays=3):
     return cs.kernel_name
 
 
-def test_graph_replay_lane_kernel(h2mech):
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_graph_replay_lane_kernel(h2mech, schedule):
     from openhurricane_amd.sampler import h2air_states
 
     t = h2mech.tables()
     st = h2air_states(h2mech.species, t["W"], 4096, seed=31)
-    name = _run(h2mech, st, kernel=1, n=4096)
+    name = _run(h2mech, st, kernel=1, n=4096, schedule=schedule)
     assert name.startswith("lane:")
 
 
-def test_graph_replay_generic_kernel():
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_graph_replay_generic_kernel(schedule):
     from openhurricane_amd.chemistry import Mechanism
     from openhurricane_amd.sampler import synthetic_workload
 
     mech = Mechanism.synthetic(32, 150, seed=1)
     t = mech.tables()
     st = synthetic_workload(mech.species, t["W"], 1024, seed=33)
-    name = _run(mech, st, kernel=2, n=1024)
+    name = _run(mech, st, kernel=2, n=1024, schedule=schedule)
     assert name.startswith("wave:")
 
 

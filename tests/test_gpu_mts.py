@@ -1,0 +1,102 @@
+"""MTS explicit Euler (integrator = 4, the reference GPU path's MTSEuler
+solver type, CUDAChemistrySourceODEs.cu:684-778) on the compiled lane kernel
+and the generic wavefront kernel against the oracle's restatement: states,
+source terms, step counts, the untouched sub-step memory.
+
+The group of a species is the integer part of log10(dt / tci), a group's
+stepping stops at the first increment within tolerance, and the fast group
+steps at the edge of explicit stability: the method itself turns
+rounding-level differences into visible ones in a few cells (the oracle run
+again from T perturbed by 1e-15 relative moves 0.7 % of 2,048 H2-air cells by
+more than 1e-6, up to 8e-3).  So the device is held to that envelope: the
+share of cells differing from the oracle by more than 1e-6 at most twice the
+oracle's own share under the perturbation (plus 0.25 %), 90 % of the cells
+within 1e-9, and the total step count within 0.1 %.
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a HIP GPU")]
+
+
+def _rel(T, Y, To, Yo):
+    """Per-cell relative difference; cells the explicit steps blew up (non-
+    finite in the oracle) count as equal when the device's are non-finite
+    too, as different otherwise."""
+    with np.errstate(invalid="ignore", over="ignore"):
+        dT = np.abs(T - To) / To
+        major = Yo >= 1e-3
+        dY = np.where(major, np.abs(Y - Yo) / np.maximum(Yo, 1e-300), 0.0).max(axis=0)
+        e = np.maximum(dT, dY)
+    bad_o = ~(np.isfinite(To) & np.isfinite(Yo).all(axis=0))
+    bad = ~(np.isfinite(T) & np.isfinite(Y).all(axis=0))
+    e = np.where(bad_o & bad, 0.0, e)
+    return np.where(bad_o != bad, np.inf, e)
+
+
+def _check(mech, orc, st, n, kernel, name_prefix, flow=0):
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource
+
+    o = _lib.default_opts(atol=1e-11, rtol=1e-4, integrator=4, kernel=kernel, flow_type=flow,
+                          max_steps=100000)
+    cs = ChemistrySource(mech, opts=o)
+    print(f"MTS {name_prefix}: {n} cells on {cs.kernel_name}", flush=True)
+    assert cs.kernel_name.startswith(name_prefix), cs.kernel_name
+    T0, p, rho = (np.ascontiguousarray(st[k][:n]) for k in ("T", "p", "rho"))
+    Y0 = np.ascontiguousarray(st["Y"][:, :n])
+    dt = np.full(n, 1e-6)
+    sub0 = np.full(n, 3.7e-8)
+    To, Yo, sub_o, s_o = orc.solve_update(o, T0, p, rho, Y0, dt, 1.0, subDt=sub0, nthreads=16)
+    T, Y, sub = T0.copy(), Y0.copy(), sub0.copy()
+    _, s = cs.solve_update(T, p, rho, Y, dt, 1.0, subDt=sub)
+    print("gpu", s, "oracle", s_o, flush=True)
+    print("cells blown up (non-finite): oracle", int((~np.isfinite(To)).sum()), "device",
+          int((~np.isfinite(T)).sum()))
+    e = _rel(T, Y, To, Yo)
+    # the algorithm's own rounding envelope
+    Tp = T0 * (1.0 + 1e-15 * np.random.default_rng(3).standard_normal(n))
+    Tq, Yq, _, _ = orc.solve_update(o, Tp, p, rho, Y0, dt, 1.0, subDt=sub0, nthreads=16)
+    env = np.mean(_rel(Tq, Yq, To, Yo) > 1e-6)
+    print("rel err: max", e.max(), "p99", np.quantile(e, 0.99), "median", np.median(e),
+          "share > 1e-6", np.mean(e > 1e-6), "oracle envelope share", env)
+    assert np.mean(e > 1e-6) <= 2 * env + 0.0025 and np.quantile(e, 0.9) <= 1e-9
+    assert np.array_equal(sub, sub0)
+    assert s["cells"] == n and s["newton_iters"] == 0
+    assert abs(s["steps"] - s_o["steps"]) <= 1e-3 * s_o["steps"] + 2 * s_o["max_cell_steps"]
+    # chemistrySource::solve form: R from the same advance
+    R, _, _ = cs.solve(T0, p, rho, Y0, dt, 1.0, subDt=sub0.copy())
+    Ro, _, _, _ = orc.solve(o, T0, p, rho, Y0, dt, 1.0, subDt=sub0.copy(), nthreads=16)
+    with np.errstate(invalid="ignore", over="ignore"):
+        eR = np.abs(R - Ro).max(axis=0) / (np.abs(Ro).max(axis=0) + 1e-300)
+    eR = np.where(~np.isfinite(Ro).all(axis=0) & ~np.isfinite(R).all(axis=0), 0.0, eR)
+    assert np.mean(~(eR <= 1e-6)) <= 2 * env + 0.0025
+
+
+@pytest.mark.parametrize("kernel,prefix", [(1, "lane:"), (2, "wave:")])
+def test_mts_h2air_matches_oracle(h2mech, h2oracle, kernel, prefix):
+    from openhurricane_amd.sampler import h2air_states
+
+    st = h2air_states(h2mech.species, h2mech.tables()["W"], 2048, seed=41)
+    _check(h2mech, h2oracle, st, 2048, kernel, prefix)
+
+
+def test_mts_h2air_constant_volume(h2mech, h2oracle):
+    from openhurricane_amd.sampler import h2air_states
+
+    st = h2air_states(h2mech.species, h2mech.tables()["W"], 512, seed=43)
+    _check(h2mech, h2oracle, st, 512, 1, "lane:", flow=1)
+
+
+def test_mts_synthetic32_matches_oracle():
+    from oracle import Oracle
+
+    from openhurricane_amd.chemistry import Mechanism
+    from openhurricane_amd.sampler import synthetic_workload
+
+    mech = Mechanism.synthetic(32, 150, seed=1)
+    st = synthetic_workload(mech.species, mech.tables()["W"], 384, seed=45, front=0.2)
+    _check(mech, Oracle(mech), st, 384, 2, "wave:")
