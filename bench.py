@@ -6,18 +6,24 @@ mechanism, 4,194,304 cells -- 20 % ignition-front states (T 1100-1600 K fresh
 mixtures), 40 % cold (300-1000 K) and 40 % burnt (pre-integrated 1e-2 s from
 1800-2600 K) -- advanced over dt = 1e-6 s by chemistrySource::solveUpdate with
 the ImplicitEuler integrator (Newton rtol 1e-6 / atol 1e-10, step error at the
-reference defaults RTolIncrement 1e-3 / ATolIncrement 1e-7) and the reference's
-Jacobian (its dKp/dT sign convention), one persistent work-queue kernel with
-the autotuned grid.  One step = one flow step on device-resident SoA fields:
-restore (T, Y) from pristine copies and advance; the per-cell sub-step memory
-(dtInit) and the cost-ordered cell schedule carry over from step to step.
+reference defaults RTolIncrement 1e-3 / ATolIncrement 1e-7), one persistent
+work-queue kernel with the autotuned grid.  The Jacobian carries the exact
+dKp/dT: with the reference's sign convention for it (reactionTypes.cpp:
+198-215) Newton needs about 75 times the iterations on this mechanism (6,300
+instead of 85 per cell), which makes a 4M-cell flow step take minutes; that
+variant is measured on 65,536 cells of the same workload in the same line
+(reference_jacobian).  One step = one flow step on device-resident SoA
+fields: restore (T, Y) from pristine copies and advance; the per-cell
+sub-step memory (dtInit) and the cost-ordered cell schedule carry over from
+step to step.
 
 The same JSON line carries, measured in the same run: the same workload with
 each step's output fed to the next (consecutive flow steps) and the first,
 schedule-less call; the 1,048,576-cell H2-air workload (with mu, kappa, D_k,
-the FP32 split and the reference's other integrators); the gated surrogate on
-16,777,216 training pairs and 4,194,304 held-out cells; and the FP64 CPU
-restatement on all host cores over a stratified 65,536-cell sample.
+the FP32 split and the reference's other integrators, with its own
+Jacobian); the gated surrogate on 16,777,216 training pairs and 4,194,304
+held-out cells; and the FP64 CPU restatement on all host cores over a
+stratified 65,536-cell sample.
 
 Prints ONE JSON line.
 """
@@ -86,29 +92,37 @@ def stratified(kind, m):
     return np.sort(sel[rng.permutation(len(sel))][:m])
 
 
-def cpu_baseline(mech, opts, st, idx, threads, label):
+def cpu_baseline(mech, opts, st, idx, threads, label, sub_all):
     """The FP64 CPU restatement of the reference algorithm (the oracle,
     compiled -O3) on `threads` host threads, one contiguous cell range per
-    thread as the reference runs one MPI rank per core; the first call sets
-    the sub-step memory, the second (a flow step like the GPU's timed ones)
-    is timed."""
+    thread as the reference runs one MPI rank per core, timed on the same
+    flow step as the GPU's timed ones: from the same states with the
+    sub-step memory the GPU carried into them (sub_all).  The sample runs in
+    a few slices, each logged, so that a long CPU leg shows progress; the
+    time is the sum of the slices."""
     sys.path.insert(0, str(ROOT / "oracle"))
     from oracle import Oracle
 
     orc = Oracle(mech)
     T, p, rho = (np.ascontiguousarray(st[k][idx]) for k in ("T", "p", "rho"))
     Y = np.ascontiguousarray(st["Y"][:, idx])
+    sub = np.ascontiguousarray(sub_all[idx])
     m = len(idx)
     dt = np.full(m, 1e-6)
-    _, _, sub, _ = orc.solve_update(opts, T, p, rho, Y, dt, 1.0, nthreads=threads)
-    t0 = time.perf_counter()
-    orc.solve_update(opts, T, p, rho, Y, dt, 1.0, subDt=sub, nthreads=threads)
-    el = time.perf_counter() - t0
-    log(f"cpu {label}: {m} cells on {threads} threads, {el:.2f} s")
+    chunk = max(threads * 64, -(-m // 6))
+    el = 0.0
+    for s0 in range(0, m, chunk):
+        sl = slice(s0, min(m, s0 + chunk))
+        args = [np.ascontiguousarray(a[sl]) for a in (T, p, rho)]
+        Ys = np.ascontiguousarray(Y[:, sl])
+        t0 = time.perf_counter()
+        orc.solve_update(opts, *args, Ys, dt[sl], 1.0, subDt=sub[sl], nthreads=threads)
+        el += time.perf_counter() - t0
+        log(f"cpu {label}: {sl.stop}/{m} cells, {el:.1f} s")
     return m / el, el
 
 
-def cpu_legs(mech, opts, st, m, m_single):
+def cpu_legs(mech, opts, st, m, m_single, sub):
     info = cpu_info()
     cores = info["available"]
     kind = st.get("kind")
@@ -120,11 +134,12 @@ def cpu_legs(mech, opts, st, m, m_single):
         idx = stratified(kind, m)
         idx1 = idx[:: max(1, len(idx) // m_single)][:m_single]
         how = f"stratified {m} cells (same front / cold / burnt shares)"
-    v, el = cpu_baseline(mech, opts, st, idx, cores, "all cores")
-    v1, el1 = cpu_baseline(mech, opts, st, idx1, 1, "one core")
+    v, el = cpu_baseline(mech, opts, st, idx, cores, "all cores", sub)
+    v1, el1 = cpu_baseline(mech, opts, st, idx1, 1, "one core", sub)
     return {"value": v, "unit": "cells/s", "cores": cores, "kind": "port",
-            "sample": f"{how} of the same seeded workload, second flow step (carried sub-step "
-                      f"memory), FP64 oracle -O3 on {cores} host threads, {el:.2f} s wall",
+            "sample": f"{how} of the same seeded workload, the timed flow step (the sub-step "
+                      f"memory the GPU carried into it), FP64 oracle -O3 on {cores} host "
+                      f"threads, {el:.2f} s",
             "single_core_cells_per_s": v1,
             "single_core_sample": f"{len(idx1)} cells of the same sample, {el1:.2f} s",
             "nproc": info["nproc"], "cpu_model": info["model"]}
@@ -238,10 +253,14 @@ def build_synthetic(name, n):
     burner = ChemistrySource(mech, opts=_lib.default_opts(**dict(SYN_TOL, jacobian=1)))
 
     def burn(T, p, rho, Y, dt):
+        log(f"burning the pool of {T.shape[0]} hot states over {dt[0]:g} s")
+        t0 = time.perf_counter()
         burner.solve_update(T, p, rho, Y, dt, 1.0)
+        log(f"burnt in {time.perf_counter() - t0:.1f} s")
 
     log(f"building {n} cells ({name})")
     st = synthetic_workload(mech.species, tab["W"], n, seed=2024, front=front, burn=burn)
+    log("cells built")
     return mech, tab, st, front
 
 
@@ -253,9 +272,10 @@ def bench_synthetic(args):
 
     ns, nr, seed, default_cells, _ = SYNTH[args.workload]
     n = args.cells or default_cells
+    jacobian = 1 if args.jacobian is None else args.jacobian
     mech, tab, st, front = build_synthetic(args.workload, n)
     dev = torch.device("cuda:0")
-    opts = _lib.default_opts(**SYN_TOL, jacobian=args.jacobian, device_pointers=1,
+    opts = _lib.default_opts(**SYN_TOL, jacobian=jacobian, device_pointers=1,
                              schedule=args.schedule)
     cs = ChemistrySource(mech, device=0, opts=opts)
     stream = torch.cuda.current_stream(dev)
@@ -269,22 +289,26 @@ def bench_synthetic(args):
     log(f"autotune {tune}")
     ms, kern_ms = timed_steps(cs, g, T0, Y0, dt, sub, args.steps, args.warmup, stream)
     stats = counters(cs, g, T0, Y0, dt, sub)
+    sub_host = sub.cpu().numpy()
     kname = f"ie_wave_kernel<{cs.kernel_name.split(':')[1]}>"
     roof = roofline(tab, stats, kern_ms, kname, f"ie_wave_{args.workload}_{n >> 20}M_pmc.txt")
     extra = {}
     if not args.no_consecutive:
         extra["consecutive_steps"] = consecutive(cs, g, T0, Y0, dt, 3, stream)
     del cs
+    if jacobian == 1 and not args.no_refjac:
+        extra["reference_jacobian"] = reference_jacobian_leg(mech, g, st, args.refjac_cells,
+                                                             stream)
     if args.workload == "syn32" and not args.no_c2:
         extra["h2air_1M"] = h2air_leg(args)
     if args.workload == "syn32" and not args.no_surrogate:
         extra["surrogate"] = surrogate_leg(args)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_legs(mech, _lib.default_opts(**SYN_TOL, jacobian=args.jacobian), st,
-                       args.cpu_sample, args.cpu_single)
-    jac = "reference Jacobian (its dKp/dT sign convention)" if args.jacobian == 0 else \
-        "exact Jacobian"
+        cpu = cpu_legs(mech, _lib.default_opts(**SYN_TOL, jacobian=jacobian), st,
+                       args.cpu_sample, args.cpu_single, sub_host)
+    jac = "reference Jacobian (its dKp/dT sign convention)" if jacobian == 0 else \
+        "exact dKp/dT in the Jacobian"
     line = {
         "metric": METRIC, "value": n / (ms * 1e-3), "unit": "cells/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
@@ -295,12 +319,52 @@ def bench_synthetic(args):
                                f"ImplicitEuler (Newton rtol 1e-6 atol 1e-10, step error "
                                f"rtol 1e-3 atol 1e-7), {jac}",
                    "cells": n, "mechanism": args.workload, "integrator": "ImplicitEuler",
-                   "jacobian": args.jacobian, "parallelism": "single GPU", "n": ns + 1,
+                   "jacobian": jacobian, "parallelism": "single GPU", "n": ns + 1,
                    "autotune": tune},
         "roofline": roof, "cpu_baseline": cpu,
     }
     line.update(extra)
     print(json.dumps(line), flush=True)
+
+
+def h2_jacobian(args):
+    """H2-air runs the reference's own Jacobian unless one is forced."""
+    return 0 if args.jacobian is None else args.jacobian
+
+
+def reference_jacobian_leg(mech, g, st, m, stream):
+    """The 32-species workload with the reference's Jacobian (its dKp/dT sign
+    convention, reactionTypes.cpp:198-215) on the first m cells: its Newton
+    iteration converges slowly where heat release couples to reversible
+    rates, so a 4M-cell flow step would take minutes; the leg reports its
+    rate and Newton iterations per cell next to the exact-Jacobian headline
+    (same converged-state tolerance)."""
+    import torch
+
+    from openhurricane_amd import _lib
+    from openhurricane_amd.chemistry import ChemistrySource
+
+    cs = ChemistrySource(mech, opts=_lib.default_opts(**SYN_TOL, jacobian=0, device_pointers=1,
+                                                      schedule=1))
+    cs.set_stream(stream.cuda_stream)
+    sl = {k: (v[:, :m] if v.dim() == 2 else v[:m]).contiguous() for k, v in g.items()}
+    dt = torch.full((m,), 1e-6, dtype=torch.float64, device=sl["T"].device)
+    sub = torch.zeros(m, dtype=torch.float64, device=sl["T"].device)
+    out = {}
+    for i in range(2):
+        T, Y = sl["T"].clone(), sl["Y"].clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, stt = cs.solve_update(T, sl["p"], sl["rho"], Y, dt, 1.0, subDt=sub, stats=True)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        log(f"reference-Jacobian leg, step {i}: {el:.2f} s")
+        out = {"cells": m, "value": m / el, "unit": "cells/s",
+               "newton_iters_per_cell": stt["newton_iters"] / m,
+               "steps_per_cell": stt["steps"] / m}
+    out["note"] = ("first cells of the same workload, second flow step with carried sub-step "
+                   "memory; the reference Jacobian's dKp/dT sign slows Newton on this mechanism")
+    return out
 
 
 def h2air_run(args, n):
@@ -318,7 +382,7 @@ def h2air_run(args, n):
     tab = mech.tables()
     st = h2air_states(mech.species, tab["W"], n, seed=2024)
     opts = _lib.default_opts(**H2_TOL, device_pointers=1, schedule=args.schedule,
-                             jacobian=args.jacobian)
+                             jacobian=h2_jacobian(args))
     cs = ChemistrySource(mech, device=0, opts=opts)
     stream = torch.cuda.current_stream(dev)
     cs.set_stream(stream.cuda_stream)
@@ -337,12 +401,12 @@ def h2air_run(args, n):
     roof = roofline(tab, stats, kern_ms, "ie_lane_kernel<Mech_h2air>",
                     "ie_lane_h2air_1M_pmc.txt")
     roof["autotune"] = tune
-    return mech, st, g, T0, Y0, dt, stream, ms, roof
+    return mech, st, g, T0, Y0, dt, stream, ms, roof, sub
 
 
 def h2air_leg(args):
     n = 1 << 20
-    mech, st, g, T0, Y0, dt, stream, ms, roof = h2air_run(args, n)
+    mech, st, g, T0, Y0, dt, stream, ms, roof, _ = h2air_run(args, n)
     out = {"workload": "H2-air 9sp/19rxn, 1,048,576 cells, dt 1e-6 s, plain FP64 ImplicitEuler "
                        "(rtol 1e-6, atol 1e-10) + mu/kappa/D_k at the new state",
            "value": n / (ms * 1e-3), "unit": "cells/s", "ms_per_step": ms, "roofline": roof}
@@ -356,13 +420,13 @@ def bench_h2air(args):
     from openhurricane_amd import _lib
 
     n = args.cells or (1 << 20)
-    mech, st, g, T0, Y0, dt, stream, ms, roof = h2air_run(args, n)
+    mech, st, g, T0, Y0, dt, stream, ms, roof, sub = h2air_run(args, n)
     split = None if args.no_split else split_precision(args, mech, g, T0, Y0, dt, stream)
     others = None if args.no_split else other_integrators(mech, g, T0, Y0, dt, stream)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_legs(mech, _lib.default_opts(**H2_TOL, jacobian=args.jacobian), st,
-                       args.cpu_sample, args.cpu_single)
+        cpu = cpu_legs(mech, _lib.default_opts(**H2_TOL, jacobian=h2_jacobian(args)), st,
+                       args.cpu_sample, args.cpu_single, sub.cpu().numpy())
     line = {
         "metric": METRIC, "value": n / (ms * 1e-3), "unit": "cells/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
@@ -389,7 +453,8 @@ def surrogate_leg(args):
         from openhurricane_amd.surrogate_workflow import run
 
         r = run(["--pairs", str(pairs), "--eval", str(held), "--epochs", str(epochs),
-                 "--peer-tol", "1e-5", "1e-4", "--jacobian", str(args.jacobian)])
+                 "--peer-tol", "1e-5", "1e-4", "--jacobian",
+                 str(1 if args.jacobian is None else args.jacobian)])
     except Exception as e:  # the plain-path line must still be printed
         return {"error": f"{type(e).__name__}: {e}"}
     gated = []
@@ -488,7 +553,7 @@ def split_precision(args, mech, g, T0, Y0, dt, stream):
     from openhurricane_amd import _lib
     from openhurricane_amd.chemistry import ChemistrySource
 
-    kw = dict(H2_TOL, device_pointers=1, schedule=args.schedule, jacobian=args.jacobian)
+    kw = dict(H2_TOL, device_pointers=1, schedule=args.schedule, jacobian=h2_jacobian(args))
     res = {}
     for prec in (0, 1):
         c = ChemistrySource(mech, device=0, opts=_lib.default_opts(precision=prec, **kw))
@@ -546,8 +611,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--schedule", type=int, default=1,
                     help="1: longest-first cell order from the previous step's per-cell cost")
-    ap.add_argument("--jacobian", type=int, default=0,
-                    help="0: reference Jacobian (its dKp/dT sign convention); 1: exact")
+    ap.add_argument("--jacobian", type=int, default=None,
+                    help="0: reference Jacobian (its dKp/dT sign convention); 1: exact dKp/dT "
+                         "(default: exact for the synthetic workloads, reference for H2-air)")
+    ap.add_argument("--refjac-cells", type=int, default=1 << 16,
+                    help="cells of the reference-Jacobian leg of the synthetic workloads")
+    ap.add_argument("--no-refjac", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the gated-surrogate leg")
     ap.add_argument("--surrogate-pairs", type=int, default=1 << 24)
     ap.add_argument("--surrogate-eval", type=int, default=1 << 22)

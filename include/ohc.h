@@ -92,6 +92,13 @@ extern "C" {
 #define OHC_RATE_THIRDBODY 1      /* ArrheniusThirdBodyRR                   */
 #define OHC_RATE_FALLOFF 2        /* unimolecularFallOffRR                  */
 #define OHC_RATE_CHEMACT 3        /* chemicallyActicatedBimolecularRR       */
+/* Read from CHEMKIN text and carried in the tables, but not evaluated on the
+ * GPU: ohc_ctx_create refuses a mechanism that holds them, as the reference's
+ * GPU table builder does ("Unsupported reaction rate type",
+ * src/speciesTransport/reactions/reactionTypes/src/reversibleReactions.cpp:186,
+ * irreversibleReactions.cpp:188, reversibleReactionWithRevPars.cpp:109). */
+#define OHC_RATE_PLOG 4           /* logInterpolationRR (PLOG)              */
+#define OHC_RATE_CHEB 5           /* ChebyshevPolynomialsRR (CHEB/TCHEB/PCHEB) */
 
 /* fall-off broadening function (fallOffFunctions) */
 #define OHC_FO_LINDEMANN 0
@@ -128,6 +135,18 @@ typedef struct ohc_mech_tables {
     const int32_t *fo_type;       /* [nr] OHC_FO_*                                */
     const double *fo;             /* [nr*5] Troe alpha,T***,T*,T** / SRI a,b,c,d,e */
     const double *eff;            /* [nr*ns] third-body efficiencies              */
+    /* Per-reaction data of the PLOG and CHEB laws: reaction r owns
+     * aux[aux_off[r] .. aux_off[r+1]) (aux_off has nr+1 entries; both may be
+     * NULL when no reaction is PLOG or CHEB).
+     *   PLOG: 4 values per pressure point, ascending in p: p [Pa], A [SI
+     *         kmol-m^3-s], beta, Ta [K] (logInterpolationRR, PLOG lines
+     *         chemkinFileRead.cpp:606-644, units :1595-1601).
+     *   CHEB: N, M, Tmin [K], Tmax [K], Pmin [Pa], Pmax [Pa], then the N*M
+     *         coefficients a(n,m) row by row (ChebyshevPolynomialsRR, CHEB /
+     *         TCHEB / PCHEB lines chemkinFileRead.cpp:645-755, units
+     *         :1623-1627; limits default to 300-2500 K and 0.001-100 atm). */
+    const int32_t *aux_off;
+    const double *aux;
 } ohc_mech_tables;
 
 /* Integrator and context options; defaults are the reference defaults. */

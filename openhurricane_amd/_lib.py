@@ -32,7 +32,7 @@ class MechTables(C.Structure):
         ("nf", _ip), ("nb", _ip), ("f_idx", _ip), ("b_idx", _ip),
         ("f_nu", _dp), ("b_nu", _dp), ("f_ord", _dp), ("b_ord", _dp),
         ("rev", _ip), ("kind", _ip), ("arr", _dp), ("arr_lo", _dp), ("arr_rev", _dp),
-        ("fo_type", _ip), ("fo", _dp), ("eff", _dp),
+        ("fo_type", _ip), ("fo", _dp), ("eff", _dp), ("aux_off", _ip), ("aux", _dp),
     ]
 
 

@@ -79,6 +79,8 @@ class Mechanism:
             setattr(t, f, dptr(f))
         for f in ["nf", "nb", "f_idx", "b_idx", "rev", "kind", "fo_type"]:
             setattr(t, f, iptr(f))
+        if tab.get("aux_off") is not None:  # PLOG / CHEB data
+            t.aux_off, t.aux = iptr("aux_off"), dptr("aux")
         h = C.c_void_p()
         check(lib().ohc_mech_from_tables(C.byref(t), C.byref(h)))
         return cls(h)
@@ -117,7 +119,22 @@ class Mechanism:
             d[f] = _as_np(getattr(t, f), nr * 3).reshape(nr, 3)
         d["fo"] = _as_np(t.fo, nr * 5).reshape(nr, 5)
         d["eff"] = _as_np(t.eff, nr * ns).reshape(nr, ns)
+        d["aux_off"] = _as_np(t.aux_off, nr + 1, np.int32)
+        d["aux"] = _as_np(t.aux, int(d["aux_off"][-1])) if d["aux_off"][-1] else np.zeros(0)
         return d
+
+    def rate_data(self, r: int) -> dict:
+        """The PLOG pressure points or the CHEB grid of reaction r, in SI units
+        (p in Pa, A in kmol-m^3-s, Ta in K); empty for the other rate laws."""
+        d = self.tables()
+        a = d["aux"][d["aux_off"][r]:d["aux_off"][r + 1]]
+        if d["kind"][r] == 4:
+            return {"law": "PLOG", "points": a.reshape(-1, 4)}
+        if d["kind"][r] == 5:
+            n, m = int(a[0]), int(a[1])
+            return {"law": "CHEB", "N": n, "M": m, "T": (a[2], a[3]), "P": (a[4], a[5]),
+                    "coef": a[6:].reshape(n, m)}
+        return {}
 
     @property
     def ns(self):

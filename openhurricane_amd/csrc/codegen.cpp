@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <sstream>
+#include <stdexcept>
 #include <string>
 
 #include "codegen.hpp"
@@ -31,6 +32,10 @@ std::string d17(double v) {
 
 std::string codegen_mechanism(const Mechanism &m, const std::string &name) {
     const int P = OHC_MAXP, ns = m.ns, nr = m.nr;
+    for (int r = 0; r < nr; ++r)
+        if (m.kind[r] > OHC_RATE_CHEMACT)
+            throw std::runtime_error("reaction " + m.rxn_names[r] +
+                                     ": PLOG/CHEB rates have no GPU kernel");
     std::ostringstream o;
     o << "// Generated from the mechanism tables by codegen.cpp -- do not edit.\n";
     o << "#pragma once\n#include \"chem_device.hpp\"\n\n";

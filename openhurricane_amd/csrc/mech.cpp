@@ -362,8 +362,16 @@ int Mechanism::add_reaction() {
     arr_rev.resize((size_t)nr * 3, 0.0);
     fo.resize((size_t)nr * 5, 0.0);
     eff.resize((size_t)nr * ns, 0.0);
+    aux_off.push_back((int32_t)aux.size());
     rxn_names.emplace_back();
     return r;
+}
+
+void Mechanism::set_aux(int r, const std::vector<double> &v) {
+    if (r != nr - 1 || aux_off[r + 1] != aux_off[r])
+        throw std::runtime_error("rate data must be attached to the last reaction once");
+    aux.insert(aux.end(), v.begin(), v.end());
+    aux_off[r + 1] = (int32_t)aux.size();
 }
 
 int Mechanism::species_index(const std::string &name) const {
@@ -403,6 +411,8 @@ ohc_mech_tables Mechanism::tables() const {
     t.fo_type = fo_type.data();
     t.fo = fo.data();
     t.eff = eff.data();
+    t.aux_off = aux_off.data();
+    t.aux = aux.data();
     return t;
 }
 
@@ -423,6 +433,7 @@ uint64_t Mechanism::hash() const {
     vd(W), vd(Tlow), vd(Thigh), vd(Tcommon), vd(nasa_lo), vd(nasa_hi);
     vi(nf), vi(nb), vi(f_idx), vi(b_idx), vd(f_nu), vd(b_nu), vd(f_ord), vd(b_ord);
     vi(rev), vi(kind), vi(fo_type), vd(arr), vd(arr_lo), vd(arr_rev), vd(fo), vd(eff);
+    if (!aux.empty()) vi(aux_off), vd(aux);  // leaves the hash of other mechanisms as it was
     return h;
 }
 
@@ -438,6 +449,29 @@ uint64_t Mechanism::hash_transport() const {
     mix(ekb);
     mix(sigma);
     return h;
+}
+
+// Layout of the PLOG / CHEB data of reaction r (ohc_mech_tables.aux).
+static void check_aux(const Mechanism &m, int r) {
+    const double *a = m.aux.data() + m.aux_off[r];
+    const int n = m.aux_off[r + 1] - m.aux_off[r];
+    const std::string where = " in reaction " + m.rxn_names[r];
+    if (m.kind[r] == OHC_RATE_PLOG) {
+        if (n < 4 || n % 4) throw std::runtime_error("PLOG needs 4 values per pressure point" + where);
+        for (int i = 0; i < n; i += 4) {
+            if (!(a[i] > 0)) throw std::runtime_error("PLOG pressure must be > 0" + where);
+            if (i && a[i] < a[i - 4]) throw std::runtime_error("PLOG pressures not ascending" + where);
+        }
+    } else if (m.kind[r] == OHC_RATE_CHEB) {
+        const bool dims = n >= 6 && a[0] >= 1 && a[1] >= 1 && a[0] <= 64 && a[1] <= 64 &&
+                          a[0] == std::floor(a[0]) && a[1] == std::floor(a[1]) &&
+                          n == 6 + (int)a[0] * (int)a[1];
+        if (!dims) throw std::runtime_error("CHEB needs N, M (1..64) and N*M coefficients" + where);
+        if (!(a[2] > 0 && a[3] > a[2])) throw std::runtime_error("TCHEB needs 0 < Tmin < Tmax" + where);
+        if (!(a[4] > 0 && a[5] > a[4])) throw std::runtime_error("PCHEB needs 0 < Pmin < Pmax" + where);
+    } else if (n != 0) {
+        throw std::runtime_error("PLOG/CHEB data on a reaction of another rate law" + where);
+    }
 }
 
 void load_from_tables(const ohc_mech_tables &t, Mechanism &m) {
@@ -497,9 +531,19 @@ void load_from_tables(const ohc_mech_tables &t, Mechanism &m) {
         for (int i = 0; i < m.nb[r]; ++i)
             if (m.b_idx[r * P + i] < 0 || m.b_idx[r * P + i] >= t.ns)
                 throw std::runtime_error("species index out of range");
-        if (m.kind[r] < 0 || m.kind[r] > OHC_RATE_CHEMACT || m.rev[r] < 0 ||
+        if (m.kind[r] < 0 || m.kind[r] > OHC_RATE_CHEB || m.rev[r] < 0 ||
             m.rev[r] > OHC_RXN_REV_PARAMS)
             throw std::runtime_error("bad reaction type code");
+        if (m.kind[r] == OHC_RATE_PLOG || m.kind[r] == OHC_RATE_CHEB) {
+            if (!t.aux_off || !t.aux)
+                throw std::runtime_error("PLOG/CHEB reaction " + std::to_string(r) +
+                                         " without aux_off/aux tables");
+            const int32_t a = t.aux_off[r], b = t.aux_off[r + 1];
+            if (a < 0 || b < a) throw std::runtime_error("aux_off not ascending");
+            m.aux.insert(m.aux.end(), t.aux + a, t.aux + b);
+        }
+        m.aux_off[r + 1] = (int32_t)m.aux.size();
+        check_aux(m, r);
     }
     for (int k = 0; k < t.ns; ++k)
         if (!(m.W[k] > 0)) throw std::runtime_error("molecular weight must be > 0");
@@ -583,6 +627,7 @@ void parse_chemkin(const std::string &chem, const std::string *thermo,
         std::string pm;
         double A = 0, beta = 0, E = 0;
         std::vector<double> low, high, troe, sri, revp;
+        std::vector<double> plog, cheb, tcheb, pcheb;
         std::map<std::string, double> effs;
         std::map<std::string, double> ford, rord;
         std::string name;
@@ -662,7 +707,18 @@ void parse_chemkin(const std::string &chem, const std::string *thermo,
                 auto nm = split_ws(g.vals);
                 if (nm.size() != 2) throw std::runtime_error("bad FORD/RORD: " + line);
                 (k == "FORD" ? p.ford : p.rord)[nm[0]] = to_double(nm[1], "order");
-            } else if (k == "PLOG" || k == "CHEB" || k == "TCHEB" || k == "PCHEB" || k == "LT" ||
+            } else if (k == "PLOG") {  // chemkinFileRead.cpp:606-644: one point per group
+                auto v = vals();
+                if (v.size() != 4) throw std::runtime_error("PLOG needs /p A beta E/ in: " + line);
+                p.plog.insert(p.plog.end(), v.begin(), v.end());
+            } else if (k == "CHEB") {  // :645-685: every CHEB group appends its numbers
+                auto v = vals();
+                p.cheb.insert(p.cheb.end(), v.begin(), v.end());
+            } else if (k == "TCHEB" || k == "PCHEB") {  // :686-755
+                auto v = vals();
+                if (v.size() != 2) throw std::runtime_error(k + " needs /min max/ in: " + line);
+                (k == "TCHEB" ? p.tcheb : p.pcheb) = v;
+            } else if (k == "LT" ||
                        k == "RLT" || k == "JAN" || k == "FIT1" || k == "HV" || k == "TDEP" ||
                        k == "EXCI" || k == "MOME" || k == "XSMI" || k == "UNITS" ||
                        k == "COLLEFF") {
@@ -729,6 +785,52 @@ void parse_chemkin(const std::string &chem, const std::string *thermo,
         const double base = 1e-3;
         m.rev[r] = !p.reversible ? OHC_RXN_IRREVERSIBLE
                                  : (p.revp.empty() ? OHC_RXN_REVERSIBLE : OHC_RXN_REV_PARAMS);
+        const bool plog = !p.plog.empty(),
+                   cheb = !p.cheb.empty() || !p.tcheb.empty() || !p.pcheb.empty();
+        if (plog || cheb) {
+            // chemkinFileRead.cpp:606-755 (reading) and :1595-1650 (units):
+            // PLOG pressures atm -> Pa, A and E as the main line; CHEB
+            // pressure limits atm -> Pa, coefficients as written.
+            if (plog && cheb) throw std::runtime_error("PLOG and CHEB in one reaction: " + p.name);
+            if (!p.low.empty() || !p.high.empty() || !p.troe.empty() || !p.sri.empty())
+                throw std::runtime_error("LOW/HIGH/TROE/SRI with a PLOG or CHEB rate: " + p.name);
+            if (!p.revp.empty())
+                throw std::runtime_error("REV cannot be used with a PLOG or CHEB rate: " + p.name);
+            m.arr[3 * r + 0] = p.A * Af;  // main line, kept as read; the law does not use it
+            m.arr[3 * r + 1] = p.beta;
+            m.arr[3 * r + 2] = p.E / Ef;
+            std::vector<double> v;
+            if (plog) {
+                if (p.has_pm)
+                    throw std::runtime_error("PLOG cannot be used in a (+M) reaction: " + p.name);
+                m.kind[r] = OHC_RATE_PLOG;
+                // ascending pressure, equal pressures in reading order
+                // (logInterpolationRR::checkSizeAndPresureRank's bubble sort)
+                std::vector<int> ord(p.plog.size() / 4);
+                for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
+                std::stable_sort(ord.begin(), ord.end(),
+                                 [&](int a, int b) { return p.plog[4 * a] < p.plog[4 * b]; });
+                for (int i : ord) {
+                    const double *q = &p.plog[4 * i];
+                    v.insert(v.end(), {q[0] * kPatm, q[1] * Af, q[2], q[3] / Ef});
+                }
+            } else {
+                if (!p.has_pm)
+                    throw std::runtime_error("CHEB needs a (+M) pressure-dependent reaction: " + p.name);
+                if (p.cheb.size() < 2)
+                    throw std::runtime_error("TCHEB/PCHEB without CHEB coefficients: " + p.name);
+                m.kind[r] = OHC_RATE_CHEB;
+                const double Tmin = p.tcheb.empty() ? 300.0 : p.tcheb[0],
+                             Tmax = p.tcheb.empty() ? 2500.0 : p.tcheb[1],
+                             Pmin = p.pcheb.empty() ? 0.001 : p.pcheb[0],
+                             Pmax = p.pcheb.empty() ? 100.0 : p.pcheb[1];
+                v = {p.cheb[0], p.cheb[1], Tmin, Tmax, Pmin * kPatm, Pmax * kPatm};
+                v.insert(v.end(), p.cheb.begin() + 2, p.cheb.end());
+            }
+            m.set_aux(r, v);
+            check_aux(m, r);
+            continue;
+        }
         if (p.has_pm) {
             if (!p.revp.empty()) throw std::runtime_error("REV with fall-off not supported: " + p.name);
             const bool chemact = !p.high.empty();

@@ -32,9 +32,13 @@ struct Mechanism {
     // reactions (OHC_MAXP slots per side)
     std::vector<int32_t> nf, nb, f_idx, b_idx, rev, kind, fo_type;
     std::vector<double> f_nu, b_nu, f_ord, b_ord, arr, arr_lo, arr_rev, fo, eff;
+    // PLOG / CHEB data: reaction r owns aux[aux_off[r], aux_off[r+1])
+    std::vector<int32_t> aux_off{0};
+    std::vector<double> aux;
 
     void resize_species(int n);
     int add_reaction();  // appends zeroed reaction slots, returns its index
+    void set_aux(int r, const std::vector<double> &v);  // r must be the last reaction
     ohc_mech_tables tables() const;
     uint64_t hash() const;            // kinetics + thermo tables
     uint64_t hash_transport() const;  // hash() combined with transport data

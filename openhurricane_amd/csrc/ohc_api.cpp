@@ -507,6 +507,17 @@ int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_
     if (o.precision != 0 && o.precision != 1) return fail(nullptr, OHC_ERR_ARG, "precision must be 0 or 1");
     if (o.jacobian != 0 && o.jacobian != 1) return fail(nullptr, OHC_ERR_ARG, "jacobian must be 0 or 1");
     if (o.kernel < 0 || o.kernel > 2) return fail(nullptr, OHC_ERR_ARG, "kernel must be 0, 1 or 2");
+    // The GPU rate laws are those of the reference's GPU tables: a PLOG or
+    // CHEB reaction stops it with "Unsupported reaction rate type"
+    // (reversibleReactions.cpp:186, irreversibleReactions.cpp:188).
+    for (int r = 0; r < c->mech.nr; ++r)
+        if (c->mech.kind[r] == OHC_RATE_PLOG || c->mech.kind[r] == OHC_RATE_CHEB)
+            return fail(nullptr, OHC_ERR_UNSUPPORTED,
+                        "unsupported reaction rate type (" +
+                            std::string(c->mech.kind[r] == OHC_RATE_PLOG ? "PLOG" : "CHEB") +
+                            ") in reaction " + c->mech.rxn_names[r] +
+                            ": the GPU path evaluates Arrhenius, third-body, fall-off and "
+                            "chemically-activated rates");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(nullptr, OHC_ERR_DEVICE, "no HIP device available");
@@ -530,9 +541,6 @@ int ohc_ctx_create(const ohc_mech *m, const ohc_opts *opts, int32_t device, ohc_
             c->lane_transport = th != 0 && ent[i].thash == th;
         }
     }
-    if (!c->lane && o.precision == 1)
-        return fail(nullptr, OHC_ERR_UNSUPPORTED,
-                    "the FP32 split precision mode needs a specialised kernel for this mechanism");
     if (o.integrator < 0 || o.integrator > 4)
         return fail(nullptr, OHC_ERR_ARG, "integrator must be 0, 1, 2, 3 or 4");
     if (o.integrator >= 2 && o.precision != 0)

@@ -77,9 +77,13 @@ hipError_t wave_kim(const KimParams &p, const WMech &m, int fp32, hipStream_t s)
 // mixture-averaged diffusivities run in lane i over j with the other
 // species' values broadcast from LDS.  Same formulas as props_lane_kernel
 // (JANAF.cpp:155-189, speciesList.cpp:208-222, kineticTheory.cpp:37-168,
-// kineticTheory.inl:32-34, transportList.cpp:309-333), FP64.
+// kineticTheory.inl:32-34, transportList.cpp:309-333).  F = double: FP64;
+// F = float (split precision): the kinetic-theory species properties,
+// Wilke's rule and the binary / mixture-averaged diffusivities in FP32, the
+// thermo, mole fractions and wavefront sums in FP64.
 constexpr int kPropWaves = 4;  // cells (wavefronts) per 256-thread block
 
+template <class F>
 __global__ __launch_bounds__(256) void props_wave_kernel(const PropParams P, const WMech M) {
     __shared__ double s_mu[kPropWaves][64], s_x[kPropWaves][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -126,16 +130,17 @@ __global__ __launch_bounds__(256) void props_wave_kernel(const PropParams P, con
         if (xt != 1.0) x *= 1.0 / xt;
     }
     // species viscosity and conductivity
-    const double ekb = S[SP_EKB * 64 + lane], sig = S[SP_SIG * 64 + lane];
-    double mu = 0.0, ka = 0.0;
+    const F ekb = (F)S[SP_EKB * 64 + lane], sig = (F)S[SP_SIG * 64 + lane];
+    const F Tf = (F)T, Wf = (F)W;
+    F mu = 0, ka = 0;
     if (sp) {
-        const double Ts = T * (1.0 / ekb);
-        const double om = 1.147 * exp(-0.145 * log(Ts)) + 1.0 / ((Ts + 0.5) * (Ts + 0.5));
-        mu = 2.6693e-6 * sqrt(W * T) / ((sig * sig) * om);
-        const double Ri = kRu / W;
-        ka = mu * Ri * (cpk / Ri + 1.25);
+        const F Ts = Tf * (F(1) / ekb);
+        const F om = F(1.147) * exp(F(-0.145) * log(Ts)) + F(1) / ((Ts + F(0.5)) * (Ts + F(0.5)));
+        mu = F(2.6693e-6) * sqrt(Wf * Tf) / ((sig * sig) * om);
+        const F Ri = F(kRu) / Wf;
+        ka = mu * Ri * ((F)cpk / Ri + F(1.25));
     }
-    s_mu[wv][lane] = mu;
+    s_mu[wv][lane] = (double)mu;
     s_x[wv][lane] = x;
     __builtin_amdgcn_wave_barrier();
     if (P.mu || P.kappa) {
@@ -144,16 +149,16 @@ __global__ __launch_bounds__(256) void props_wave_kernel(const PropParams P, con
             num = mu;
             nk = ka;
         } else if (sp) {
-            double phi = 0.0;
+            F phi = 0;
             for (int j = 0; j < ns; ++j) {
-                const double Wj = S[SP_W * 64 + j];
-                const double phiP = (1.0 / sqrt(8.0)) / sqrt(1.0 + W / Wj);
-                const double tt = 1.0 + sqrt(mu / s_mu[wv][j] * sqrt(Wj / W));
-                phi += phiP * s_x[wv][j] * (tt * tt);
+                const F Wj = (F)S[SP_W * 64 + j];
+                const F phiP = (F(1) / sqrt(F(8))) / sqrt(F(1) + Wf / Wj);
+                const F tt = F(1) + sqrt(mu / (F)s_mu[wv][j] * sqrt(Wj / Wf));
+                phi += phiP * (F)s_x[wv][j] * (tt * tt);
             }
-            const double rphi = 1.0 / phi;
-            num = mu * x * rphi;
-            nk = ka * x * rphi;
+            const double rphi = 1.0 / (double)phi;
+            num = (double)mu * x * rphi;
+            nk = (double)ka * x * rphi;
         }
         const double mum = wsum(num), km = wsum(nk);
         if (lane == 0) {
@@ -162,19 +167,20 @@ __global__ __launch_bounds__(256) void props_wave_kernel(const PropParams P, con
         }
     }
     if (P.Dk && sp) {
-        const double Tv1p5 = T * sqrt(T), rp = 1.0 / p;
-        double sum = 0.0;
+        const F Tv1p5 = Tf * sqrt(Tf), rp = F(1) / (F)p;
+        F sum = 0;
         for (int j = 0; j < ns; ++j) {
             if (j == lane) continue;
-            const double Wj = S[SP_W * 64 + j], sj = S[SP_SIG * 64 + j], ej = S[SP_EKB * 64 + j];
-            const double sij = 0.5 * (sig + sj);
-            const double cst = 1.858e-7 * sqrt(1.0 / W + 1.0 / Wj) / (sij * sij) * kPatm;
-            const double Ts = T * (1.0 / sqrt(ekb * ej));
-            const double Od = exp(-0.145 * log(Ts)) + 1.0 / ((Ts + 0.5) * (Ts + 0.5));
-            const double Dij = Tv1p5 * cst * rp / Od;
-            sum += s_x[wv][j] / Dij;
+            const F Wj = (F)S[SP_W * 64 + j], sj = (F)S[SP_SIG * 64 + j],
+                    ej = (F)S[SP_EKB * 64 + j];
+            const F sij = F(0.5) * (sig + sj);
+            const F cst = F(1.858e-7) * sqrt(F(1) / Wf + F(1) / Wj) / (sij * sij) * F(kPatm);
+            const F Ts = Tf * (F(1) / sqrt(ekb * ej));
+            const F Od = exp(F(-0.145) * log(Ts)) + F(1) / ((Ts + F(0.5)) * (Ts + F(0.5)));
+            const F Dij = Tv1p5 * cst * rp / Od;
+            sum += (F)s_x[wv][j] / Dij;
         }
-        double Dim = x < 1.0 - kVeryTiny ? (1.0 - x) / sum : 0.0;
+        double Dim = x < 1.0 - kVeryTiny ? (1.0 - x) / (double)sum : 0.0;
         if (isinf(Dim)) Dim = 0.0;
         P.Dk[lane * P.ldD + cell] = Dim;
     }
@@ -229,7 +235,10 @@ __global__ __launch_bounds__(256) void tfromh_table_kernel(const ThParams P, con
 
 hipError_t wave_props(const PropParams &p, const WMech &m, hipStream_t s) {
     const unsigned grid = (unsigned)((p.n + kPropWaves - 1) / kPropWaves);
-    hipLaunchKernelGGL(props_wave_kernel, dim3(grid), dim3(64 * kPropWaves), 0, s, p, m);
+    if (p.prec == 1)
+        hipLaunchKernelGGL(props_wave_kernel<float>, dim3(grid), dim3(64 * kPropWaves), 0, s, p, m);
+    else
+        hipLaunchKernelGGL(props_wave_kernel<double>, dim3(grid), dim3(64 * kPropWaves), 0, s, p, m);
     return hipGetLastError();
 }
 

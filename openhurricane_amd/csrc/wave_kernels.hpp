@@ -148,9 +148,14 @@ __device__ __forceinline__ double xpow(const TFac &tf, double s) {
     return pow_ool(tf.X, s);
 }
 // ArrheniusRR::k: A T^b exp(-Ta/T); with b = 0 or Ta = 0 the single exp gives
-// the same value as the reference's skipped pow / exp.
+// the same value as the reference's skipped pow / exp.  F = float: the
+// split-precision mode's FP32 rate constant.
+template <class F = double>
 __device__ __forceinline__ double rarrh(double A, double b, double Ta, const TFac &tf) {
-    return A * exp(b * tf.lnT - Ta * tf.invT);
+    if constexpr (std::is_same_v<F, float>)
+        return (double)((float)A * __expf((float)b * (float)tf.lnT - (float)Ta * (float)tf.invT));
+    else
+        return A * exp(b * tf.lnT - Ta * tf.invT);
 }
 __device__ __forceinline__ double rarrh_dkdT(double k, double b, double Ta, const TFac &tf) {
     return k * tf.invTm * (b + Ta * tf.invTm);
@@ -235,7 +240,9 @@ __device__ double rfo_DFDT(int fot, const double *fp, double T, double Pr, doubl
 }
 
 // Molar G, dG/dT, Cp, H, dCp/dT of species k (JANAF.cpp:142-219), same
-// operation order as species_G / species_CH of the lane kernels.
+// operation order as species_G / species_CH of the lane kernels.  F = float:
+// G and dG/dT in FP32 (split-precision mode; Cp, H, dCp/dT stay FP64).
+template <class F = double>
 __device__ __forceinline__ void wthermo(const WMech &M, int k, const TFac &tf, double &G,
                                         double &dG, double &Cp, double &H, double &dCp) {
     const double *S = M.sp;
@@ -255,11 +262,25 @@ __device__ __forceinline__ void wthermo(const WMech &M, int k, const TFac &tf, d
     const bool out = (T < Tl) || (T > Th);
     const double T0 = out ? fmin(fmax(T, Tl), Th) : T;
     const double cp0 = ((((a[4] * T0 + a[3]) * T0 + a[2]) * T0 + a[1]) * T0 + a[0]);
-    const double s0 = ((((s4 * T + s3) * T + s2) * T + a[1]) * T + a[0] * tf.lnT + a[6]);
-    const double g0 = (((((-g4 * T - g3) * T - g2) * T - g1) * T + a[0] * (1.0 - tf.lnT) - a[6]) * T + a[5]);
-    const double ds0 = (a[0] * tf.invTm + a[1] + (a[2] + (a[3] + a[4] * T) * T) * T);
-    G = g0 * W;
-    dG = (cp0 - (s0 + T * ds0)) * W;
+    if constexpr (std::is_same_v<F, float>) {
+        const float t = (float)T, t0 = (float)T0, lnT = (float)tf.lnT, iTm = (float)tf.invTm;
+        float b[7];
+#pragma unroll
+        for (int c = 0; c < 7; ++c) b[c] = (float)a[c];
+        const float c0 = ((((b[4] * t0 + b[3]) * t0 + b[2]) * t0 + b[1]) * t0 + b[0]);
+        const float s0 = (((((float)s4 * t + (float)s3) * t + (float)s2) * t + b[1]) * t + b[0] * lnT + b[6]);
+        const float g0 = (((((-(float)g4 * t - (float)g3) * t - (float)g2) * t - (float)g1) * t +
+                           b[0] * (1.0f - lnT) - b[6]) * t + b[5]);
+        const float ds0 = (b[0] * iTm + b[1] + (b[2] + (b[3] + b[4] * t) * t) * t);
+        G = (double)(g0 * (float)W);
+        dG = (double)((c0 - (s0 + t * ds0)) * (float)W);
+    } else {
+        const double s0 = ((((s4 * T + s3) * T + s2) * T + a[1]) * T + a[0] * tf.lnT + a[6]);
+        const double g0 = (((((-g4 * T - g3) * T - g2) * T - g1) * T + a[0] * (1.0 - tf.lnT) - a[6]) * T + a[5]);
+        const double ds0 = (a[0] * tf.invTm + a[1] + (a[2] + (a[3] + a[4] * T) * T) * T);
+        G = g0 * W;
+        dG = (cp0 - (s0 + T * ds0)) * W;
+    }
     const double Tx = out ? (T < Tl ? Tl : Th) : T;
     double h = ((((h4 * Tx + h3) * Tx + h2) * Tx + h1) * Tx + a[0]) * Tx + a[5];
     if (out) h += cp0 * (T - Tx);
@@ -278,7 +299,10 @@ struct WRxn {
     double fnu[kWP], bnu[kWP];
 };
 
-template <bool WITH_J>
+// F = float: the split-precision mode -- Arrhenius rate constants and the
+// equilibrium constant (Gibbs sums and exponential) in FP32; concentration
+// products, rates, derivatives in FP64.
+template <bool WITH_J, class FT = double>
 __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf, const double *sc,
                                           const double *sG, const double *sdG, int exact,
                                           WRxn &o) {
@@ -315,12 +339,12 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
     for (int i = 0; i < 5; ++i) fp[i] = fop[i * nrp];
     const int fot = RI[RI_FOT * nrp + r];
     if (kind == 0) {
-        kf = rarrh(A, b, Ta, tf);
+        kf = rarrh<FT>(A, b, Ta, tf);
     } else if (kind == 1) {
-        kf = Mtb * rarrh(A, b, Ta, tf);
+        kf = Mtb * rarrh<FT>(A, b, Ta, tf);
     } else {
-        k0 = rarrh(RD[RD_A0 * nrp + r], RD[RD_B0 * nrp + r], RD[RD_TA0 * nrp + r], tf);
-        kinf = rarrh(A, b, Ta, tf);
+        k0 = rarrh<FT>(RD[RD_A0 * nrp + r], RD[RD_B0 * nrp + r], RD[RD_TA0 * nrp + r], tf);
+        kinf = rarrh<FT>(A, b, Ta, tf);
         const double kinfz = kinf != 0 ? kinf : kVeryTiny;
         Pr = k0 / kinfz * Mtb;
         F = rfo_F(fot, fp, tf.T, Pr);
@@ -329,25 +353,28 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
     double kr = 0.0, rK = 0.0, dKc = 0.0;
     if (rev == 1) {
         // reactionTypes::Kp / Kc / DKcDT (reactionTypes.cpp:178-241)
-        double Kt = 0.0, dk = 0.0;
+        FT Kt = 0.0, dk = 0.0;
+        const FT iTm = (FT)tf.invTm;
 #pragma unroll
         for (int i = 0; i < kWP; ++i) {
             if (i < nf) {
-                const double g = sG[fi[i]];
-                Kt -= g * fnu[i];
-                if (WITH_J) dk -= (sdG[fi[i]] - g * tf.invTm) * fnu[i];
+                const FT g = (FT)sG[fi[i]];
+                Kt -= g * (FT)fnu[i];
+                if (WITH_J) dk -= ((FT)sdG[fi[i]] - g * iTm) * (FT)fnu[i];
             }
         }
 #pragma unroll
         for (int i = 0; i < kWP; ++i) {
             if (i < nb) {
-                const double g = sG[bi[i]];
-                Kt += g * bnu[i];
-                if (WITH_J) dk += (sdG[bi[i]] - g * tf.invTm) * bnu[i];
+                const FT g = (FT)sG[bi[i]];
+                Kt += g * (FT)bnu[i];
+                if (WITH_J) dk += ((FT)sdG[bi[i]] - g * iTm) * (FT)bnu[i];
             }
         }
-        Kt *= tf.invRuT;
-        double Kp = exp(-Kt);
+        Kt *= (FT)tf.invRuT;
+        double Kp;
+        if constexpr (std::is_same_v<FT, float>) Kp = (double)__expf(-Kt);
+        else Kp = exp(-Kt);
         if (bad(Kp)) Kp = kRootVeryLarge;
         const double sS = RD[RD_SUMS * nrp + r];
         const bool zero_s = fabs(sS) < kVeryTiny;
@@ -361,9 +388,9 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
         rK = frcp(fmax(Kc, kVeryTiny));
         kr = Kc > kVeryTiny ? kf * rK : 0.0;
         if (WITH_J) {
-            dk *= tf.invRuT;
+            dk *= (FT)tf.invRuT;
             if (exact) dk = -dk;
-            double dKp = Kp * dk;
+            double dKp = Kp * (double)dk;
             if (bad(dKp)) dKp = kVeryLarge;
             if (zero_s) {
                 dKc = dKp;
@@ -374,7 +401,7 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
             }
         }
     } else if (rev == 2) {
-        kr = rarrh(RD[RD_AR * nrp + r], RD[RD_BR * nrp + r], RD[RD_TAR * nrp + r], tf);
+        kr = rarrh<FT>(RD[RD_AR * nrp + r], RD[RD_BR * nrp + r], RD[RD_TAR * nrp + r], tf);
         if (kind == 1) kr = Mtb * kr;
     }
     // concentration powers c^ord of every participant, computed once and
@@ -470,7 +497,7 @@ struct WaveLDS {
 
 // dy/dt (and the Jacobian into L.J when WITH_J) at the wavefront-distributed
 // state yv (lane i holds y_i; lane ns holds T).  Returns f_i in lane i.
-template <int N, int FLOW, bool WITH_J>
+template <int N, int FLOW, bool WITH_J, class F = double>
 __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, double yv, int exact) {
     const int lane = threadIdx.x, ns = M.ns;
     const double T = rdlane(yv, ns);
@@ -478,7 +505,7 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
     if (lane < ns) {
         L.c[lane] = fmax(yv, 0.0);
         double G, dG, Cp, H, dCp;
-        wthermo(M, lane, tf, G, dG, Cp, H, dCp);
+        wthermo<F>(M, lane, tf, G, dG, Cp, H, dCp);
         L.G[lane] = G;
         L.dG[lane] = dG;
         L.Cp[lane] = Cp;
@@ -494,7 +521,7 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
         double gam = 0.0;
         if (r < M.nr) {
             WRxn o;
-            wreaction<WITH_J>(M, r, tf, L.c, L.G, L.dG, exact, o);
+            wreaction<WITH_J, F>(M, r, tf, L.c, L.G, L.dG, exact, o);
             // d(omega)/dt rows of the participants: sigma_a q, and with J the
             // row terms sigma_a dq/dc_j and sigma_a dq/dT
 #pragma unroll
@@ -673,7 +700,10 @@ __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, 
 // on steps 4 and 5, error y - y0 - h f(y) on the first three steps and the
 // third difference of the step-start states after, and an attempt whose error
 // grew from the previous attempt's but stays below 5 accepted as it is.
-template <int N, int FLOW, int METH = 0>
+// F = float: the split-precision mode (FP32 Gibbs energies, equilibrium and
+// Arrhenius rate constants; FP64 concentration products, net rates,
+// Jacobian, LU, error norms and sums), ImplicitEuler only.
+template <int N, int FLOW, int METH = 0, class F = double>
 __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const WaveParams WP) {
     const LaneParams &P = WP.L;
     const WMech &M = WP.M;
@@ -794,7 +824,7 @@ __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const Wa
                     double lastErr = 0.0, maxErr = 0.0;
                     bool conv = true;
                     for (;;) {
-                        const double f = wave_eval<N, FLOW, true>(M, L, yk, WP.exact_dkp);
+                        const double f = wave_eval<N, FLOW, true, F>(M, L, yk, WP.exact_dkp);
                         ++c_newt;
                         ++citer;
                         const double h = METH == 2 ? 0.5 * dtl : dtl;
@@ -870,7 +900,7 @@ __global__ __launch_bounds__(64, (N <= 40 ? 3 : 2)) void ie_wave_kernel(const Wa
                             }
                         } else if (nstep == 0) {
                             ++c_rhs;
-                            const double f = wave_eval<N, FLOW, false>(M, L, yk, WP.exact_dkp);
+                            const double f = wave_eval<N, FLOW, false, F>(M, L, yk, WP.exact_dkp);
                             e = yk - y0 - dtl * f;
                         } else {
                             const double dtn = lts0, dtn1 = lts1;
@@ -1251,7 +1281,7 @@ __global__ __launch_bounds__(64) void kim_wave_kernel(const KimParams P, const W
 }
 
 // dy/dt and J at given states, one wavefront per cell.
-template <int N, int FLOW>
+template <int N, int FLOW, class F = double>
 __global__ __launch_bounds__(64) void eval_wave_kernel(const EvalParams P, const WMech M, int exact) {
     __shared__ WaveLDS<N> L;
     const int lane = threadIdx.x, ns = M.ns, n = ns + 1;
@@ -1261,8 +1291,8 @@ __global__ __launch_bounds__(64) void eval_wave_kernel(const EvalParams P, const
     const double pr = P.pr[cell];
     (void)pr;
     double f;
-    if (P.J) f = wave_eval<N, FLOW, true>(M, L, y, exact);
-    else f = wave_eval<N, FLOW, false>(M, L, y, exact);
+    if (P.J) f = wave_eval<N, FLOW, true, F>(M, L, y, exact);
+    else f = wave_eval<N, FLOW, false, F>(M, L, y, exact);
     if (lane < n) P.f[lane * P.ldf + cell] = f;
     if (P.J && lane < n)
         for (int j = 0; j < n; ++j) P.J[(lane * n + j) * P.ldJ + cell] = L.J[lane * WaveLDS<N>::JS + j];

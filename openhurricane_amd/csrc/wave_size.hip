@@ -22,6 +22,9 @@ hipError_t OHC_CAT(wave_integrate_, OHC_WAVE_N)(const WaveParams &p, int grid, h
     } else if (p.L.integrator == 2) {
         if (p.L.flow == 0) hipLaunchKernelGGL((ie_wave_kernel<N, 0, 2>), dim3(grid), dim3(64), 0, s, p);
         else hipLaunchKernelGGL((ie_wave_kernel<N, 1, 2>), dim3(grid), dim3(64), 0, s, p);
+    } else if (p.L.prec == 1) {
+        if (p.L.flow == 0) hipLaunchKernelGGL((ie_wave_kernel<N, 0, 0, float>), dim3(grid), dim3(64), 0, s, p);
+        else hipLaunchKernelGGL((ie_wave_kernel<N, 1, 0, float>), dim3(grid), dim3(64), 0, s, p);
     } else if (p.L.flow == 0) {
         hipLaunchKernelGGL((ie_wave_kernel<N, 0>), dim3(grid), dim3(64), 0, s, p);
     } else {
@@ -34,8 +37,14 @@ hipError_t OHC_CAT(wave_eval_, OHC_WAVE_N)(const EvalParams &p, const WMech &m, 
                                             hipStream_t s) {
     constexpr int N = OHC_WAVE_N;
     const dim3 g((unsigned)p.n);
-    if (p.flow == 0) hipLaunchKernelGGL((eval_wave_kernel<N, 0>), g, dim3(64), 0, s, p, m, exact);
-    else hipLaunchKernelGGL((eval_wave_kernel<N, 1>), g, dim3(64), 0, s, p, m, exact);
+    if (p.prec == 1) {
+        if (p.flow == 0) hipLaunchKernelGGL((eval_wave_kernel<N, 0, float>), g, dim3(64), 0, s, p, m, exact);
+        else hipLaunchKernelGGL((eval_wave_kernel<N, 1, float>), g, dim3(64), 0, s, p, m, exact);
+    } else if (p.flow == 0) {
+        hipLaunchKernelGGL((eval_wave_kernel<N, 0>), g, dim3(64), 0, s, p, m, exact);
+    } else {
+        hipLaunchKernelGGL((eval_wave_kernel<N, 1>), g, dim3(64), 0, s, p, m, exact);
+    }
     return hipGetLastError();
 }
 

@@ -54,6 +54,10 @@ static int *dup_i(const int32_t *s, size_t n) {
 }
 
 orc_mech *orc_mech_new(const ohc_mech_tables *t) {
+    /* The restated rate laws are those of the reference's GPU tables
+     * (reversibleReactions.cpp:64-187): PLOG and CHEB are not among them. */
+    for (int r = 0; r < t->nr; ++r)
+        if (t->kind[r] < OHC_RATE_ARRHENIUS || t->kind[r] > OHC_RATE_CHEMACT) return NULL;
     orc_mech *m = (orc_mech *)calloc(1, sizeof(orc_mech));
     const int ns = t->ns, nr = t->nr, P = OHC_MAXP;
     m->ns = ns;

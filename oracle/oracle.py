@@ -81,6 +81,9 @@ class Oracle:
         self.ns = t.ns
         self.nr = t.nr
         self._h = lib().orc_mech_new(C.byref(t))
+        if not self._h:
+            raise ValueError("the oracle restates Arrhenius, third-body, fall-off and "
+                             "chemically-activated rates only (no PLOG/CHEB)")
 
     def __del__(self):
         try:

@@ -132,6 +132,34 @@ def test_generic_synthetic_solve_matches_oracle(syn32, syn32_oracle, flow, exact
     assert np.max(np.abs(R - Ro) / scale) <= 1e-5
 
 
+def test_generic_burnt_states_match_oracle(syn32, syn32_oracle):
+    """The benchmark's burnt cells: fresh hot mixtures (1800-2600 K)
+    integrated over 1e-2 s (large sub-steps, where I - h J needs pivoting
+    away from the diagonal), then the burnt states over one 1e-6 s flow
+    step, with the reference Jacobian: states and Newton counts of the
+    oracle."""
+    from openhurricane_amd.chemistry import ChemistrySource
+    from openhurricane_amd.sampler import fresh_states, mixture_density
+
+    t = syn32.tables()
+    n = 64
+    b = fresh_states(syn32.species, t["W"], n, np.random.Generator(np.random.PCG64(81)),
+                     (1800.0, 2600.0))
+    for exact, span in ((1, 1e-2), (0, 1e-6)):
+        o = _opts(jacobian=exact, atol_inc=1e-7, rtol_inc=1e-3)
+        cs = ChemistrySource(syn32, opts=o)
+        dt = np.full(n, span)
+        print(f"burnt states: {n} cells over {span} s, jacobian {exact}", flush=True)
+        To, Yo, _, s_o = syn32_oracle.solve_update(o, b["T"], b["p"], b["rho"], b["Y"], dt, 1.0,
+                                                   nthreads=16)
+        T, Y = b["T"].copy(), b["Y"].copy()
+        _, s = cs.solve_update(T, b["p"], b["rho"], Y, dt, 1.0)
+        print("gpu", s, "oracle", s_o, flush=True)
+        _check_states(T, Y, To, Yo)
+        assert abs(s["newton_iters"] - s_o["newton_iters"]) <= 0.02 * s_o["newton_iters"]
+        b = {"T": To, "p": b["p"], "rho": mixture_density(b["p"], To, Yo, t["W"]), "Y": Yo}
+
+
 def test_generic_schedule_is_bitwise_neutral(syn32):
     from openhurricane_amd.chemistry import ChemistrySource
     from openhurricane_amd.sampler import synthetic_workload
@@ -390,3 +418,26 @@ def test_energy_forms_match_oracle(which, h2mech, h2oracle, syn32, syn32_oracle)
     np.testing.assert_allclose(Tg, To, rtol=1e-12)
     np.testing.assert_array_equal(fl, flo)
     np.testing.assert_allclose(Tg, st["T"], rtol=1e-4)
+
+
+@pytest.mark.parametrize("kind", ["h2air", "syn32"])
+def test_generic_split_precision_against_oracle(kind):
+    """The FP32 split on the generic (run-time table) kernels: errors against
+    the FP64 oracle per output, bounded on the non-runaway cells."""
+    from oracle import Oracle
+
+    from openhurricane_amd.chemistry import Mechanism
+    from openhurricane_amd.sampler import h2air_states, synthetic_workload
+    from test_gpu_parity import split_precision_check
+
+    if kind == "h2air":
+        mech = Mechanism.h2air()
+        st = h2air_states(mech.species, mech.tables()["W"], 1024, seed=71)
+        tol = dict(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6, kernel=2)
+        n = 1024
+    else:
+        mech = Mechanism.synthetic(32, 150, seed=1)
+        st = synthetic_workload(mech.species, mech.tables()["W"], 256, seed=72, front=0.2)
+        tol = dict(atol=1e-10, rtol=1e-6, atol_inc=1e-7, rtol_inc=1e-3, jacobian=1)
+        n = 256
+    split_precision_check(mech, Oracle(mech), st, n, tol, "wave:")

