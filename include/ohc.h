@@ -283,6 +283,26 @@ int ohc_chem_solve_update(ohc_ctx *ctx, int64_t n, double *T, const double *p,
                           const double *rho, double *Y, int64_t ldY, const double *dt,
                           double dtFactor, double *subDt, ohc_stats *stats);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  ohc_chem_solve_update on device arrays (the context must
+ * have device_pointers = 1) that allocates nothing: the integrator's work-queue
+ * head, its counters and the longest-first schedule live in a static device
+ * array of the library (33.6 MB on each device it runs on, held until the
+ * process ends) instead of buffers the library allocates.  n <= 
+ * OHC_NOALLOC_MAX_CELLS, larger n is rejected with OHC_ERR_ARG.  With
+ * opts.schedule the cells are taken longest-first by the Newton counts of the
+ * previous call on this device when it had the same n (else in index order);
+ * results do not depend on the order.  All calls on one device share the
+ * array: calls that overlap in time on one device race (serialise them, e.g.
+ * on one stream).  stats (host, nullable): when given, the call synchronises
+ * the stream and reports failed cells as OHC_ERR_NUMERIC; without it the call
+ * is asynchronous and performs no host synchronisation. */
+#define OHC_NOALLOC_MAX_CELLS 4194304
+int ohc_chem_solve_update_noalloc(ohc_ctx *ctx, int64_t n, double *T, const double *p,
+                                  const double *rho, double *Y, int64_t ldY, const double *dt,
+                                  double dtFactor, double *subDt, ohc_stats *stats);
+
 /* calculateSourceTerms(false) (diagJ == NULL) and calculateSourceTermsImp
  * (diagJ != NULL, diagonal d(omega_k)/d(c_k) incl. the temperature coupling
  * term; withoutLastSpec selects the omegaCoupled last-species substitution). */

@@ -97,7 +97,10 @@ _SIGS = [
     ("ohc_mix_thermo", C.c_int, [_V, C.c_int64, _V, _V, _V, C.c_int64, _V, _V, _V, _V]),
     ("ohc_chem_source_kim", C.c_int, [_V, C.c_int64, _V, _V, _V, _V, C.c_int64, _V, C.c_int64,
                                       _V, C.c_int64, C.c_int32]),
+    ("ohc_chem_solve_update_noalloc", C.c_int, [_V, C.c_int64, _V, _V, _V, _V, C.c_int64, _V,
+                                                C.c_double, _V, C.POINTER(Stats)]),
 ]
+OHC_NOALLOC_MAX_CELLS = 4194304
 
 _FP = C.POINTER(C.c_float)
 _SIGS += [

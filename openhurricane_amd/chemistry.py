@@ -267,6 +267,20 @@ class ChemistrySource:
         check(rc, self._h)
         return subDt, (st.as_dict() if st else None)
 
+    # solveUpdate without any device allocation (ohc_chem_solve_update_noalloc):
+    # device-pointer contexts only, n <= OHC_NOALLOC_MAX_CELLS
+    def solve_update_noalloc(self, T, p, rho, Y, dt, dtFactor=1.0, subDt=None, stats=True):
+        n = T.shape[0]
+        if subDt is None:
+            subDt = self._zeros(n)
+        st = Stats() if stats else None
+        rc = lib().ohc_chem_solve_update_noalloc(self._h, n, self._p(T), self._p(p),
+                                                 self._p(rho), self._p(Y), _ld(Y, n),
+                                                 self._p(dt), float(dtFactor), self._p(subDt),
+                                                 C.byref(st) if st else None)
+        check(rc, self._h)
+        return subDt, (st.as_dict() if st else None)
+
     # chemistrySource::DyDt / jacobian at states y (N, n)
     def eval_rhs(self, y, p):
         n = y.shape[1]
