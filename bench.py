@@ -43,6 +43,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "chem cells/s per flow step on 1 GPU; surrogate vs plain at ≤1e-3 rel err"
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
+HBM_PEAK_GBS = 8000.0    # MI355X HBM3E
 SYNTH = {"syn32": (32, 150, 1, 1 << 22, 0.2), "syn54": (54, 270, 2, 1 << 23, 0.3)}
 # step-error tolerances of the synthetic workloads: the reference defaults
 SYN_TOL = dict(atol=1e-10, rtol=1e-6, atol_inc=1e-7, rtol_inc=1e-3)
@@ -401,7 +402,34 @@ def h2air_run(args, n):
     roof = roofline(tab, stats, kern_ms, "ie_lane_kernel<Mech_h2air>",
                     "ie_lane_h2air_1M_pmc.txt")
     roof["autotune"] = tune
+    roof["props"] = props_roofline(props, T0, Y0, mech.ns, stream)
     return mech, st, g, T0, Y0, dt, stream, ms, roof, sub
+
+
+def props_roofline(props, T, Y, ns, stream, reps=5):
+    """HBM roofline of the mixture-property kernel (mu, kappa, D_k at given
+    T, p, Y): algorithmic bytes = (ns + 2) x 8 B read + (ns + 2) x 8 B
+    written per cell, over its HIP-event time on the context stream."""
+    import torch
+
+    n = T.shape[0]
+    props(T, Y)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(reps)]
+    for a, b in evs:
+        a.record(stream)
+        props(T, Y)
+        b.record(stream)
+    torch.cuda.synchronize()
+    ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    byts = n * (2 * ns + 4) * 8
+    gbs = byts / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "props_lane_kernel<Mech_h2air>", "kernel_ms": ms, "bytes_per_launch": byts,
+            "note": "T, p, Y in and mu, kappa, D_k out per cell; the kernel's Wilke and "
+                    "mixture-averaged diffusion sums (ns^2 terms with square roots) make it "
+                    "compute-bound well below this line"}
 
 
 def h2air_leg(args):

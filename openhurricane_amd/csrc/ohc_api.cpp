@@ -102,7 +102,8 @@ int ensure_sched(ohc_ctx *c, int64_t n) {
 }
 
 // Device tables of the generic kernels: species fields [field][64], reaction
-// fields [field][nrp] and efficiencies [nr][64].
+// fields chunk-major [nrp / 64][field][64] (the 64 reactions a wavefront
+// evaluates at once, field by field) and efficiencies [nr][64].
 int build_wmech(ohc_ctx *c) {
     using namespace ohcd;
     const ohc::Mechanism &m = c->mech;
@@ -156,37 +157,39 @@ int build_wmech(ohc_ctx *c) {
         auto key = [&](int r) { return m.kind[r] * 100 + m.rev[r] * 10 + m.fo_type[r]; };
         return key(a) < key(b);
     });
+    auto xi = [](int f, int q) { return ((size_t)(q / 64) * RI_NFIELD + f) * 64 + q % 64; };
+    auto xd = [](int f, int q) { return ((size_t)(q / 64) * RD_NFIELD + f) * 64 + q % 64; };
     std::vector<int> ri((size_t)RI_NFIELD * nrp, 0);
     std::vector<double> rd((size_t)RD_NFIELD * nrp, 0.0), eff((size_t)std::max(nr, 1) * 64, 0.0);
     for (int q = 0; q < nr; ++q) {
         const int r = perm[q];
-        ri[RI_NF * nrp + q] = m.nf[r];
-        ri[RI_NB * nrp + q] = m.nb[r];
-        ri[RI_REV * nrp + q] = m.rev[r];
-        ri[RI_KIND * nrp + q] = m.kind[r];
-        ri[RI_FOT * nrp + q] = m.fo_type[r];
+        ri[xi(RI_NF, q)] = m.nf[r];
+        ri[xi(RI_NB, q)] = m.nb[r];
+        ri[xi(RI_REV, q)] = m.rev[r];
+        ri[xi(RI_KIND, q)] = m.kind[r];
+        ri[xi(RI_FOT, q)] = m.fo_type[r];
         double s = 0.0;
         for (int i = 0; i < m.nf[r]; ++i) {
             const int k = m.f_idx[r * P + i];
-            ri[(RI_FI + i) * nrp + q] = k;
-            rd[(RD_FNU + i) * nrp + q] = m.f_nu[r * P + i];
-            rd[(RD_FO + i) * nrp + q] = m.f_ord[r * P + i];
+            ri[xi(RI_FI + i, q)] = k;
+            rd[xd(RD_FNU + i, q)] = m.f_nu[r * P + i];
+            rd[xd(RD_FO + i, q)] = m.f_ord[r * P + i];
             s -= m.f_nu[r * P + i];
         }
         for (int i = 0; i < m.nb[r]; ++i) {
             const int k = m.b_idx[r * P + i];
-            ri[(RI_BI + i) * nrp + q] = k;
-            rd[(RD_BNU + i) * nrp + q] = m.b_nu[r * P + i];
-            rd[(RD_BO + i) * nrp + q] = m.b_ord[r * P + i];
+            ri[xi(RI_BI + i, q)] = k;
+            rd[xd(RD_BNU + i, q)] = m.b_nu[r * P + i];
+            rd[xd(RD_BO + i, q)] = m.b_ord[r * P + i];
             s += m.b_nu[r * P + i];
         }
         for (int i = 0; i < 3; ++i) {
-            rd[(RD_A + i) * nrp + q] = m.arr[3 * r + i];
-            rd[(RD_A0 + i) * nrp + q] = m.arr_lo[3 * r + i];
-            rd[(RD_AR + i) * nrp + q] = m.arr_rev[3 * r + i];
+            rd[xd(RD_A + i, q)] = m.arr[3 * r + i];
+            rd[xd(RD_A0 + i, q)] = m.arr_lo[3 * r + i];
+            rd[xd(RD_AR + i, q)] = m.arr_rev[3 * r + i];
         }
-        for (int i = 0; i < 5; ++i) rd[(RD_FOP + i) * nrp + q] = m.fo[5 * r + i];
-        rd[RD_SUMS * nrp + q] = s;
+        for (int i = 0; i < 5; ++i) rd[xd(RD_FOP + i, q)] = m.fo[5 * r + i];
+        rd[xd(RD_SUMS, q)] = s;
         if (m.kind[r] != OHC_RATE_ARRHENIUS)
             for (int k = 0; k < ns; ++k) eff[(size_t)q * 64 + k] = m.eff[(size_t)r * ns + k];
     }

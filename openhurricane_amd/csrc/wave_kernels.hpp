@@ -30,6 +30,12 @@
 namespace ohcd {
 
 constexpr int kWP = 3;  // distinct species per reaction side in the run-time tables
+
+// 1: the LU's pivot row reaches the other lanes by v_readlane pairs (no LDS
+// round trip and no barrier per elimination step); 0: through LDS.
+#ifndef OHC_LU_LANE_READS
+#define OHC_LU_LANE_READS 1
+#endif
 static_assert(kWP == 3, "the reaction kernels multiply three participant powers");
 
 // species fields, [field][64]
@@ -50,9 +56,12 @@ enum {
     SP_SIG,               // transport: sigma [Angstrom]
     SP_NFIELD
 };
-// reaction int fields, [field][nrp]
+// Reaction tables are chunk-major: reaction r = 64 c + l (the chunk a
+// wavefront evaluates at once, l its lane) keeps field f at
+// [(c * NFIELD + f) * 64 + l].
+// reaction int fields
 enum { RI_NF, RI_NB, RI_FI, RI_BI = RI_FI + kWP, RI_REV = RI_BI + kWP, RI_KIND, RI_FOT, RI_NFIELD };
-// reaction double fields, [field][nrp]
+// reaction double fields
 enum {
     RD_FNU,
     RD_BNU = RD_FNU + kWP,
@@ -76,8 +85,8 @@ enum {
 struct WMech {
     int ns, nr, nrp;      // nrp = nr rounded up to a multiple of 64
     const double *sp;     // [SP_NFIELD][64]
-    const int *ri;        // [RI_NFIELD][nrp]
-    const double *rd;     // [RD_NFIELD][nrp]
+    const int *ri;        // [nrp / 64][RI_NFIELD][64]
+    const double *rd;     // [nrp / 64][RD_NFIELD][64]
     const double *eff;    // [nr][64] third-body efficiencies
 };
 
@@ -86,6 +95,40 @@ struct WaveParams {
     WMech M;
     int exact_dkp;  // 1: exact d(Kp)/dT instead of the reference's sign convention
 };
+
+// Reads of the per-reaction tables through buffer resources: the lane is
+// the one vector offset, the chunk and field a scalar offset, so a table read
+// needs no vector address arithmetic.  Reads past a table return zero.
+struct RxnTab {
+    __amdgpu_buffer_rsrc_t ri, rd, ef;
+};
+__device__ __forceinline__ RxnTab rxn_tab(const WMech &M) {
+    constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit data, CDNA
+    RxnTab t;
+    t.ri = __builtin_amdgcn_make_buffer_rsrc((void *)M.ri, 0, RI_NFIELD * M.nrp * 4, kRsrcWord3);
+    t.rd = __builtin_amdgcn_make_buffer_rsrc((void *)M.rd, 0, RD_NFIELD * M.nrp * 8, kRsrcWord3);
+    t.ef = __builtin_amdgcn_make_buffer_rsrc((void *)M.eff, 0, M.nr * 64 * 8, kRsrcWord3);
+    return t;
+}
+// field f of reaction 64 c + lane (c wave-uniform)
+__device__ __forceinline__ int ldi(const RxnTab &t, int f, int c, int lane) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(t.ri, lane * 4, (c * RI_NFIELD + f) * 256, 0);
+}
+__device__ __forceinline__ double ldd(const RxnTab &t, int f, int c, int lane) {
+    return __builtin_bit_cast(
+        double, __builtin_amdgcn_raw_buffer_load_b64(t.rd, lane * 8, (c * RD_NFIELD + f) * 512, 0));
+}
+// efficiency of species k (wave-uniform) in reaction r
+__device__ __forceinline__ double lde(const RxnTab &t, int r, int k) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(t.ef, r * 512, k * 8, 0));
+}
+// field f of reaction r for a wave-uniform r (scalar reads)
+__device__ __forceinline__ int ri_at(const WMech &M, int f, int r) {
+    return M.ri[((r >> 6) * RI_NFIELD + f) * 64 + (r & 63)];
+}
+__device__ __forceinline__ double rd_at(const WMech &M, int f, int r) {
+    return M.rd[((r >> 6) * RD_NFIELD + f) * 64 + (r & 63)];
+}
 
 __device__ __forceinline__ double rdlane(double v, int l) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
@@ -120,6 +163,15 @@ __device__ __forceinline__ double wmax(double v) {
     v = fmax(v, dpp<kDppHalfMirror>(v));
     v = fmax(v, dpp<kDppMirror>(v));
     return fmax(fmax(rdlane(v, 0), rdlane(v, 16)), fmax(rdlane(v, 32), rdlane(v, 48)));
+}
+__device__ __forceinline__ unsigned wmax_u32(unsigned v) {
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, kDppXor1, 0xf, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, kDppXor2, 0xf, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, kDppHalfMirror, 0xf, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, kDppMirror, 0xf, 0xf, false));
+    const unsigned a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16),
+                   c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+    return max(max(a, b), max(c, d));
 }
 
 // LDS floating-point add (ds_add_f64), result unused.
@@ -161,82 +213,69 @@ __device__ __forceinline__ double rarrh_dkdT(double k, double b, double Ta, cons
     return k * tf.invTm * (b + Ta * tf.invTm);
 }
 
-// fallOffFunctions.cpp:54-168 with run-time type and parameters.
-__device__ double rfo_F(int fot, const double *fp, double T, double Pr) {
-    if (fot == 3) {
-        const double logPr = log10(fmax(Pr, kTiny));
-        const double X = 1.0 / (1.0 + logPr * logPr);
-        return fp[3] * pow(fp[0] * exp(-fp[1] / T) + exp(-T / fp[2]), X) * pow(T, fp[4]);
-    }
-    if (fot == 1 || fot == 2) {
-        double fc = (1.0 - fp[0]) * exp(-T / fp[1]) + fp[0] * exp(-T / fp[2]);
-        if (fot == 2) fc += exp(-fp[3] / T);
-        const double logFcent = log10(fmax(fc, kTiny));
-        const double c = -0.4 - 0.67 * logFcent, n = 0.75 - 1.27 * logFcent, dd = 0.14;
-        const double logPr = log10(fmax(Pr, kTiny));
-        const double q = (logPr + c) / (n - dd * (logPr + c));
-        return pow(10.0, logFcent / (1.0 + q * q));
-    }
-    return 1.0;
-}
-__device__ double rfo_DFDci(int fot, const double *fp, double T, double Pr, double F, double dPr) {
-    if (fot == 3) {
-        const double lp = log10(fmax(Pr, kTiny));
-        const double X = 1.0 / (1.0 + lp * lp);
-        const double dXdc = -X * X * 2.0 * log10(Pr) / Pr / log(10.0) * dPr;
-        return F * dXdc * log(fp[0] * exp(-fp[1] / T) + exp(-T / fp[2]));
-    }
-    if (fot == 1 || fot == 2) {
-        double fc = (1.0 - fp[0]) * exp(-T / fp[1]) + fp[0] * exp(-T / fp[2]);
-        if (fot == 2) fc += exp(-fp[3] / T);
-        const double logFcent = log10(fmax(fc, kTiny));
-        const double c = -0.4 - 0.67 * logFcent, n = 0.75 - 1.27 * logFcent, dd = 0.14;
-        const double logPrpc = log10(fmax(Pr, kTiny)) + c;
-        const double nmd = n - dd * logPrpc;
-        const double inv = logPrpc / nmd;
-        const double q = 1.0 + inv * inv;
-        return F * ((-logFcent) / (q * q)) * (2.0 * logPrpc / (nmd * nmd)) * (1.0 + dd * inv) *
-               (1.0 / Pr * dPr);
-    }
-    return 0.0;
-}
-__device__ double rfo_DFDT(int fot, const double *fp, double T, double Pr, double F, double dPrdT) {
-    if (fot == 3) {
+// Fall-off broadening (fallOffFunctions.cpp:54-168: F, DFDci, DFDT) for a
+// run-time type and parameters, evaluated once per reaction together with
+// the pieces both derivatives reuse:
+//   dF/dc_i / F = gPr dPr/dc_i,   dF/dT / F = gT + gPr dPr/dT
+// (the reference's DFDci and DFDT divided by F; both are linear in the
+// derivative of Pr).  The published Troe and SRI forms as the reference
+// writes them, with log10 x = ln x / ln 10, 10^x = exp(x ln 10) and
+// a^X b^e = exp(X ln a + e ln b): one logarithm of Pr and of the centre /
+// SRI base shared by F and its derivatives.
+struct FoState {
+    double F, gPr, gT;
+};
+template <bool WITH_J>
+__device__ __forceinline__ FoState rfo_eval(int fot, const double *fp, const TFac &tf, double Pr) {
+    constexpr double kLn10 = 2.302585092994045684;
+    constexpr double kRln10 = 0.4342944819032518277;  // 1 / ln 10
+    FoState o{1.0, 0.0, 0.0};
+    if (fot == 0) return o;  // Lindemann
+    const double T = tf.T;
+    const double rPr = frcp(Pr);
+    const double lnPr = log(fmax(Pr, kTiny));
+    const double logPr = lnPr * kRln10;
+    if (fot == 3) {  // SRI: d [a exp(-b/T) + exp(-T/c)]^X T^e, X = 1 / (1 + log10(Pr)^2)
         const double a = fp[0], b = fp[1], c = fp[2], e = fp[4];
-        const double lp = log10(fmax(Pr, kTiny));
-        const double X = 1.0 / (1.0 + lp * lp);
-        const double dXdPr = -X * X * 2.0 * log10(Pr) / Pr / log(10.0);
-        const double ea = exp(-b / T), ec = exp(-T / c);
-        return F * (e / T + dXdPr * dPrdT * log(a * ea + ec) +
-                    X * (a * b * ea / (T * T) - ec / c) / (a * ea + ec));
-    }
-    if (fot == 1 || fot == 2) {
-        const double ln10 = log(10.0);
-        const double logPr = log10(fmax(Pr, kTiny));
-        const double e1 = exp(-T / fp[1]), e2 = exp(-T / fp[2]);
-        double Fc = (1.0 - fp[0]) * e1 + fp[0] * e2;
-        double dFc = (1.0 - fp[0]) * e1 / (-fp[1]) + fp[0] * e2 / (-fp[2]);
-        if (fot == 2) {
-            const double e3 = exp(-fp[3] / T);
-            Fc += e3;
-            dFc += e3 * (fp[3] / (T * T));
+        const double X = frcp(1.0 + logPr * logPr);
+        const double ea = exp(-b * tf.invT), ec = exp(-T / c);
+        const double base = a * ea + ec;
+        const double lnb = log(base);
+        o.F = fp[3] * exp(X * lnb + e * tf.lnT);
+        if (WITH_J) {
+            const double dXdPr = -X * X * 2.0 * logPr * rPr * kRln10;
+            o.gPr = dXdPr * lnb;
+            o.gT = e * tf.invT + X * (a * b * ea * tf.invT * tf.invT - ec / c) * frcp(base);
         }
-        Fc = fmax(Fc, kTiny);
-        const double logFcent = log10(Fc);
-        const double dlF = 1.0 / (Fc * ln10) * dFc;
-        const double c = -0.4 - 0.67 * logFcent, dc = -0.67 * dlF;
-        const double n = 0.75 - 1.27 * logFcent, dn = -1.27 * dlF;
-        const double dd = 0.14;
-        const double dlogPr = dPrdT / Pr / ln10;
-        const double lpc = logPr + c;
-        const double nmd = n - dd * lpc;
-        const double inv = lpc / nmd;
-        const double q = 1.0 + inv * inv;
-        const double dPP = 2.0 * lpc / (nmd * nmd) *
-                           ((dlogPr + dc) - lpc / nmd * (dn - dd * (dlogPr + dc)));
-        return F * ln10 * (dlF / q - logFcent / (q * q) * dPP);
+        return o;
     }
-    return 0.0;
+    // Troe, three or four parameters: Fcent, c, n, d = 0.14
+    const double e1 = exp(-T / fp[1]), e2 = exp(-T / fp[2]);
+    double Fc = (1.0 - fp[0]) * e1 + fp[0] * e2;
+    double dFc = (1.0 - fp[0]) * e1 / (-fp[1]) + fp[0] * e2 / (-fp[2]);
+    if (fot == 2) {
+        const double e3 = exp(-fp[3] * tf.invT);
+        Fc += e3;
+        dFc += e3 * (fp[3] * tf.invT * tf.invT);
+    }
+    Fc = fmax(Fc, kTiny);
+    const double lnFc = log(Fc);
+    const double logFc = lnFc * kRln10;
+    const double c = -0.4 - 0.67 * logFc, n = 0.75 - 1.27 * logFc, dd = 0.14;
+    const double lpc = logPr + c;
+    const double rnmd = frcp(n - dd * lpc);
+    const double inv = lpc * rnmd;
+    const double rq = frcp(1.0 + inv * inv);
+    o.F = exp(lnFc * rq);  // 10^(log10 Fcent / (1 + inv^2))
+    if (WITH_J) {
+        // d(log10 F)/d(log10 Pr + c) and the T derivatives of log10 Fcent, c, n
+        const double k = -logFc * rq * rq * 2.0 * lpc * rnmd * rnmd;
+        const double dlF = dFc * frcp(Fc) * kRln10;
+        const double dc = -0.67 * dlF, dn = -1.27 * dlF;
+        o.gPr = k * (1.0 + dd * inv) * rPr;
+        o.gT = kLn10 * (dlF * rq + k * (dc * (1.0 + dd * inv) - inv * dn));
+    }
+    return o;
 }
 
 // Molar G, dG/dT, Cp, H, dCp/dT of species k (JANAF.cpp:142-219), same
@@ -303,51 +342,52 @@ struct WRxn {
 // equilibrium constant (Gibbs sums and exponential) in FP32; concentration
 // products, rates, derivatives in FP64.
 template <bool WITH_J, class FT = double>
-__device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf, const double *sc,
-                                          const double *sG, const double *sdG, int exact,
-                                          WRxn &o) {
-    const int nrp = M.nrp;
-    const int *RI = M.ri;
-    const double *RD = M.rd;
-    const int nf = RI[RI_NF * nrp + r], nb = RI[RI_NB * nrp + r];
-    const int rev = RI[RI_REV * nrp + r], kind = RI[RI_KIND * nrp + r];
+__device__ __forceinline__ void wreaction(const WMech &M, int base, const TFac &tf,
+                                          const double *sc, const double *sG, const double *sdG,
+                                          int exact, WRxn &o) {
+    const RxnTab tb = rxn_tab(M);
+    const int lane = threadIdx.x, ch = base >> 6, r = base + lane;
+    const int nf = ldi(tb, RI_NF, ch, lane), nb = ldi(tb, RI_NB, ch, lane);
+    const int rev = ldi(tb, RI_REV, ch, lane), kind = ldi(tb, RI_KIND, ch, lane);
     int fi[kWP], bi[kWP];
     double fnu[kWP], bnu[kWP], fo[kWP], bo[kWP], cf[kWP], cb[kWP];
 #pragma unroll
     for (int i = 0; i < kWP; ++i) {
-        fi[i] = RI[(RI_FI + i) * nrp + r];
-        bi[i] = RI[(RI_BI + i) * nrp + r];
-        fnu[i] = RD[(RD_FNU + i) * nrp + r];
-        bnu[i] = RD[(RD_BNU + i) * nrp + r];
-        fo[i] = RD[(RD_FO + i) * nrp + r];
-        bo[i] = RD[(RD_BO + i) * nrp + r];
+        fi[i] = ldi(tb, RI_FI + i, ch, lane);
+        bi[i] = ldi(tb, RI_BI + i, ch, lane);
+        fnu[i] = ldd(tb, RD_FNU + i, ch, lane);
+        bnu[i] = ldd(tb, RD_BNU + i, ch, lane);
+        fo[i] = ldd(tb, RD_FO + i, ch, lane);
+        bo[i] = ldd(tb, RD_BO + i, ch, lane);
         cf[i] = i < nf ? sc[fi[i]] : 1.0;
         cb[i] = i < nb ? sc[bi[i]] : 1.0;
     }
-    const double A = RD[RD_A * nrp + r], b = RD[RD_B * nrp + r], Ta = RD[RD_TA * nrp + r];
+    const double A = ldd(tb, RD_A, ch, lane), b = ldd(tb, RD_B, ch, lane),
+                 Ta = ldd(tb, RD_TA, ch, lane);
     // third-body concentration, summed in species order (thirdBodyEfficiency::M)
     double Mtb = 0.0;
     if (kind != 0) {
-        const double *ef = M.eff + (size_t)r * 64;
 #pragma unroll 8
-        for (int k = 0; k < M.ns; ++k) Mtb += ef[k] * sc[k];
+        for (int k = 0; k < M.ns; ++k) Mtb += lde(tb, r, k) * sc[k];
     }
     double kf, k0 = 0.0, kinf = 0.0, Pr = 0.0, F = 1.0;
-    const double *fop = RD + RD_FOP * nrp + r;  // stride nrp
+    FoState fs{1.0, 0.0, 0.0};
     double fp[5];
 #pragma unroll
-    for (int i = 0; i < 5; ++i) fp[i] = fop[i * nrp];
-    const int fot = RI[RI_FOT * nrp + r];
+    for (int i = 0; i < 5; ++i) fp[i] = ldd(tb, RD_FOP + i, ch, lane);
+    const int fot = ldi(tb, RI_FOT, ch, lane);
     if (kind == 0) {
         kf = rarrh<FT>(A, b, Ta, tf);
     } else if (kind == 1) {
         kf = Mtb * rarrh<FT>(A, b, Ta, tf);
     } else {
-        k0 = rarrh<FT>(RD[RD_A0 * nrp + r], RD[RD_B0 * nrp + r], RD[RD_TA0 * nrp + r], tf);
+        k0 = rarrh<FT>(ldd(tb, RD_A0, ch, lane), ldd(tb, RD_B0, ch, lane),
+                       ldd(tb, RD_TA0, ch, lane), tf);
         kinf = rarrh<FT>(A, b, Ta, tf);
         const double kinfz = kinf != 0 ? kinf : kVeryTiny;
         Pr = k0 / kinfz * Mtb;
-        F = rfo_F(fot, fp, tf.T, Pr);
+        fs = rfo_eval<WITH_J>(fot, fp, tf, Pr);
+        F = fs.F;
         kf = kind == 2 ? kinf * (Pr / (1.0 + Pr)) * F : k0 * (1.0 / (1.0 + Pr)) * F;
     }
     double kr = 0.0, rK = 0.0, dKc = 0.0;
@@ -376,7 +416,7 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
         if constexpr (std::is_same_v<FT, float>) Kp = (double)__expf(-Kt);
         else Kp = exp(-Kt);
         if (bad(Kp)) Kp = kRootVeryLarge;
-        const double sS = RD[RD_SUMS * nrp + r];
+        const double sS = ldd(tb, RD_SUMS, ch, lane);
         const bool zero_s = fabs(sS) < kVeryTiny;
         double Xs = 1.0, Kc;
         if (zero_s) {
@@ -401,7 +441,8 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
             }
         }
     } else if (rev == 2) {
-        kr = rarrh<FT>(RD[RD_AR * nrp + r], RD[RD_BR * nrp + r], RD[RD_TAR * nrp + r], tf);
+        kr = rarrh<FT>(ldd(tb, RD_AR, ch, lane), ldd(tb, RD_BR, ch, lane),
+                       ldd(tb, RD_TAR, ch, lane), tf);
         if (kind == 1) kr = Mtb * kr;
     }
     // concentration powers c^ord of every participant, computed once and
@@ -446,8 +487,8 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
     } else if (kind >= 2 && Mtb > kTiny) {
         const double kinfz = kinf != 0 ? kinf : kVeryTiny;
         const double dPr = k0 / kinfz;
-        const double dF = rfo_DFDci(fot, fp, tf.T, Pr, F, dPr);
-        const double g = kind == 2 ? dPr / (Pr * (1 + Pr)) + dF / F : -dPr / (1 + Pr) + dF / F;
+        const double dlF = fs.gPr * dPr;  // (dF/dc) / F
+        const double g = kind == 2 ? dPr / (Pr * (1 + Pr)) + dlF : -dPr / (1 + Pr) + dlF;
         gam = bad(g) ? 0.0 : q * g;
     }
     o.gam = gam;
@@ -455,19 +496,22 @@ __device__ __forceinline__ void wreaction(const WMech &M, int r, const TFac &tf,
     double dkf, dkr = 0.0;
     if (kind <= 1) dkf = rarrh_dkdT(kf, b, Ta, tf);
     else if (kind == 2) dkf = (Pr / (1.0 + Pr)) * F * rarrh_dkdT(kinf, b, Ta, tf);
-    else dkf = (1.0 / (1.0 + Pr)) * F * rarrh_dkdT(k0, RD[RD_B0 * nrp + r], RD[RD_TA0 * nrp + r], tf);
+    else
+        dkf = (1.0 / (1.0 + Pr)) * F *
+              rarrh_dkdT(k0, ldd(tb, RD_B0, ch, lane), ldd(tb, RD_TA0, ch, lane), tf);
     if (rev == 1) dkr = (dkf - kr * dKc) * rK;
-    else if (rev == 2) dkr = rarrh_dkdT(kr, RD[RD_BR * nrp + r], RD[RD_TAR * nrp + r], tf);
+    else if (rev == 2)
+        dkr = rarrh_dkdT(kr, ldd(tb, RD_BR, ch, lane), ldd(tb, RD_TAR, ch, lane), tf);
     dkf *= prf;
     dkr *= prb;
     double dqdT = dkf - dkr, dgdT = 0.0;
     if (kind >= 2 && Mtb > kTiny) {
         const double k0z = k0 != 0 ? k0 : kVeryTiny, kinfz = kinf != 0 ? kinf : kVeryTiny;
         const double dPrdT =
-            Pr * (rarrh_dkdT(k0, RD[RD_B0 * nrp + r], RD[RD_TA0 * nrp + r], tf) / k0z -
+            Pr * (rarrh_dkdT(k0, ldd(tb, RD_B0, ch, lane), ldd(tb, RD_TA0, ch, lane), tf) / k0z -
                   rarrh_dkdT(kinf, b, Ta, tf) / kinfz);
-        const double dFdT = rfo_DFDT(fot, fp, tf.T, Pr, F, dPrdT);
-        const double gg = kind == 2 ? dPrdT / (Pr * (1 + Pr)) + dFdT / F : -dPrdT / (1 + Pr) + dFdT / F;
+        const double dlFdT = fs.gT + fs.gPr * dPrdT;  // (dF/dT) / F
+        const double gg = kind == 2 ? dPrdT / (Pr * (1 + Pr)) + dlFdT : -dPrdT / (1 + Pr) + dlFdT;
         dgdT = q * gg;
     }
     if (bad(dqdT)) dqdT = 0.0;
@@ -486,13 +530,12 @@ struct WaveLDS {
     double J[N * JS];
     double c[N], G[N], dG[N], Cp[N], H[N], dCp[N];  // species k in [k]
     double w[N];      // net production rates omega_k
-    double gam[64];   // third-body / fall-off coefficient of each reaction lane of a chunk
-    double dinv[N];   // 1 / pivot of each elimination step
+#if !OHC_LU_LANE_READS
     // pivot row of the current elimination step (element j at [2 j], b at
     // [2 N]): the stride keeps the writes single 64-bit stores straight from
     // the row's registers instead of packed 128-bit stores that need copies
     double prow[2 * N + 1];
-    int perm[N];      // pivot lane of each elimination step
+#endif
 };
 
 // dy/dt (and the Jacobian into L.J when WITH_J) at the wavefront-distributed
@@ -519,9 +562,10 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
     for (int base = 0; base < M.nr; base += 64) {
         const int r = base + lane;
         double gam = 0.0;
+        WRxn o;
+        o.nf = o.nb = 0;
         if (r < M.nr) {
-            WRxn o;
-            wreaction<WITH_J, F>(M, r, tf, L.c, L.G, L.dG, exact, o);
+            wreaction<WITH_J, F>(M, base, tf, L.c, L.G, L.dG, exact, o);
             // d(omega)/dt rows of the participants: sigma_a q, and with J the
             // row terms sigma_a dq/dc_j and sigma_a dq/dT
 #pragma unroll
@@ -549,32 +593,32 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
         if (WITH_J) {
             // third-body / fall-off terms sigma_a gam_r eff_rj, column j in lane
             // j: each lane updates only its own column, so plain read-modify-
-            // write suffices, one reaction of the chunk after another
+            // write suffices, one reaction of the chunk after another; the
+            // reaction's participants come from its lane's registers (the
+            // wavefront's LDS operations complete in program order, after the
+            // atomics above)
             unsigned long long tbm = __ballot(gam != 0.0);
-            if (tbm) {
-                L.gam[lane] = gam;
-                __syncthreads();
-                const int nrp = M.nrp;
-                while (tbm) {
-                    const int sl = __ffsll((long long)tbm) - 1;
-                    tbm &= tbm - 1ull;
-                    const int rr = base + sl;
-                    const double g = L.gam[sl];
-                    const double e = lane < ns ? M.eff[(size_t)rr * 64 + lane] : 0.0;
-                    const int nf = M.ri[RI_NF * nrp + rr], nb = M.ri[RI_NB * nrp + rr];
+            while (tbm) {
+                const int sl = __ffsll((long long)tbm) - 1;
+                tbm &= tbm - 1ull;
+                const int rr = base + sl;
+                const double g = rdlane(gam, sl);
+                const double e = lane < ns ? M.eff[(size_t)rr * 64 + lane] : 0.0;
+                const int nf = __builtin_amdgcn_readlane(o.nf, sl),
+                          nb = __builtin_amdgcn_readlane(o.nb, sl);
 #pragma unroll
-                    for (int i = 0; i < kWP; ++i) {
-                        if (i < nf && lane < ns) {
-                            double *pj = &L.J[M.ri[(RI_FI + i) * nrp + rr] * WaveLDS<N>::JS + lane];
-                            *pj += -M.rd[(RD_FNU + i) * nrp + rr] * g * e;
-                        }
-                        if (i < nb && lane < ns) {
-                            double *pj = &L.J[M.ri[(RI_BI + i) * nrp + rr] * WaveLDS<N>::JS + lane];
-                            *pj += M.rd[(RD_BNU + i) * nrp + rr] * g * e;
-                        }
+                for (int i = 0; i < kWP; ++i) {
+                    if (i < nf && lane < ns) {
+                        const int k = __builtin_amdgcn_readlane(o.fi[i], sl);
+                        double *pj = &L.J[k * WaveLDS<N>::JS + lane];
+                        *pj += -rdlane(o.fnu[i], sl) * g * e;
+                    }
+                    if (i < nb && lane < ns) {
+                        const int k = __builtin_amdgcn_readlane(o.bi[i], sl);
+                        double *pj = &L.J[k * WaveLDS<N>::JS + lane];
+                        *pj += rdlane(o.bnu[i], sl) * g * e;
                     }
                 }
-                __syncthreads();
             }
         }
     }
@@ -615,18 +659,20 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
 }
 
 // Solve (diag I - h J) x = b with the Jacobian rows in LDS: lane i owns row i;
-// partial pivoting as SquareMatrix::LUDecomposeDPivoting (largest |pivot|),
-// forward substitution folded into the elimination.  Returns x_i in lane i.
-// The elimination steps are unrolled at compile time, so step k touches only
-// the N - k - 1 columns right of the pivot.  The pivot row reaches the other
-// lanes through LDS (OHC_LU_LANE_READS = 1: through v_readlane pairs).
-#ifndef OHC_LU_LANE_READS
-#define OHC_LU_LANE_READS 0
-#endif
-template <int N>
-__device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, double b,
-                                                double diag = 1.0) {
+// partial pivoting as SquareMatrix::LUDecomposeDPivoting (largest |pivot|,
+// lowest row among equals), forward substitution folded into the
+// elimination.  Returns x_i in lane i.  The elimination steps are unrolled at
+// compile time, so step k touches only the N - k - 1 columns right of the
+// pivot; FULL (n == N) drops the per-step bounds, so that the whole
+// factorisation is one straight-line block the compiler can schedule across
+// steps.  The pivot row reaches the other lanes through v_readlane pairs
+// (OHC_LU_LANE_READS = 0: through LDS); the pivot row and reciprocal of each
+// step are kept in lane k of two registers for the back substitution.
+template <int N, bool FULL>
+__device__ __forceinline__ double wave_lu_solve_n(WaveLDS<N> &L, int n, double h, double b,
+                                                  double diag) {
     const int lane = threadIdx.x;
+    if (FULL) n = N;
     const bool act = lane < n;
     // every lane loads a row that exists (lanes past the system read row 0
     // and discard it), so the N loads issue back to back without a branch
@@ -636,27 +682,36 @@ __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, 
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         const double x = Jr[j];
-        const double v = act && j < n ? -h * x : 0.0;
+        const double v = act && (FULL || j < n) ? -h * x : 0.0;
         a[j] = (act && j == lane) ? v + diag : v;
     }
     int step = act ? -1 : N;  // elimination step at which this row became the pivot
+    int permv = 0;            // lane k: pivot row of step k
+    double dinvv = 0.0;       // lane k: 1 / pivot of step k
     sfor<0, N>([&](auto kc) {
         constexpr int k = kc;
-        if (k < n) {
-            const double v = step < 0 ? fabs(a[k]) : -1.0;
-            const double m = wmax(v);
-            const unsigned long long hit = __ballot(step < 0 && v == m);
+        if (FULL || k < n) {
+            // largest |a_ik| over the rows not yet used, lowest row among
+            // equals: the maximum of the high words of |a_ik|, then of the
+            // low words among the rows that reach it (two 32-bit reductions
+            // give the exact order of non-negative doubles)
+            const double av = fabs(a[k]);
+            const bool cand = step < 0 && !isnan(av);
+            const unsigned hi = cand ? (unsigned)__double2hiint(av) : 0u;
+            const unsigned mhi = wmax_u32(hi);
+            const bool top = cand && hi == mhi;
+            const unsigned lo = top ? (unsigned)__double2loint(av) : 0u;
+            const unsigned mlo = wmax_u32(lo);
+            const unsigned long long hit = __ballot(top && lo == mlo);
             const int p = hit ? (int)__ffsll((long long)hit) - 1 : 0;  // no hit only for NaN rows
             const double pk = rdlane(a[k], p);
             const double rp = frcp(pk);
-            if (lane == 0) {
-                L.perm[k] = p;
-                L.dinv[k] = rp;
-            }
+            permv = lane == k ? p : permv;
+            dinvv = lane == k ? rp : dinvv;
             const bool el = step < 0 && lane != p;
-            if (lane == p) step = k;
+            step = lane == p ? k : step;
             const double l = el ? a[k] * rp : 0.0;
-            if (el) a[k] = l;
+            a[k] = el ? l : a[k];
             // columns j >= n hold zeros in every row and stay zero, so the
             // update needs no per-column predicate
 #if OHC_LU_LANE_READS
@@ -678,13 +733,20 @@ __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, 
     double x = 0.0;
     sfor<0, N>([&](auto kk) {
         constexpr int k = N - 1 - kk;
-        if (k < n) {
-            const double xk = rdlane(b, L.perm[k]) * L.dinv[k];
-            if (step < k) b = fma(-a[k], xk, b);
-            if (lane == k) x = xk;
+        if (FULL || k < n) {
+            const int pk = __builtin_amdgcn_readlane(permv, k);
+            const double xk = rdlane(b, pk) * rdlane(dinvv, k);
+            b = step < k ? fma(-a[k], xk, b) : b;
+            x = lane == k ? xk : x;
         }
     });
     return x;
+}
+template <int N>
+__device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, double b,
+                                                double diag = 1.0) {
+    return n == N ? wave_lu_solve_n<N, true>(L, n, h, b, diag)
+                  : wave_lu_solve_n<N, false>(L, n, h, b, diag);
 }
 
 // Persistent integrator: each wavefront takes one cell at a time from the
@@ -1038,7 +1100,7 @@ __device__ __forceinline__ double wave_mts_diag(const WMech &M, WaveLDS<N> &L, d
         const int r = base + lane;
         if (r < M.nr) {
             WRxn o;
-            wreaction<true>(M, r, tf, L.c, L.G, L.dG, exact, o);
+            wreaction<true>(M, base, tf, L.c, L.G, L.dG, exact, o);
 #pragma unroll
             for (int i = 0; i < kWP; ++i) {
                 if (i < o.nf) {
@@ -1234,7 +1296,7 @@ __global__ __launch_bounds__(64) void kim_wave_kernel(const KimParams P, const W
         const int r = base + lane;
         if (r < M.nr) {
             WRxn o;
-            wreaction<true>(M, r, tf, L.c, L.G, L.dG, P.exact, o);
+            wreaction<true>(M, base, tf, L.c, L.G, L.dG, P.exact, o);
 #pragma unroll
             for (int i = 0; i < kWP; ++i) {
                 if (i < o.nf) lds_add(&L.w[o.fi[i]], -o.fnu[i] * o.q);
