@@ -103,7 +103,8 @@ int ensure_sched(ohc_ctx *c, int64_t n) {
 
 // Device tables of the generic kernels: species fields [field][64], reaction
 // fields chunk-major [nrp / 64][field][64] (the 64 reactions a wavefront
-// evaluates at once, field by field) and efficiencies [nr][64].
+// evaluates at once, field by field) -- a packed 32-bit table read for every
+// reaction and an FP64 table -- and efficiencies [nr][64].
 int build_wmech(ohc_ctx *c) {
     using namespace ohcd;
     const ohc::Mechanism &m = c->mech;
@@ -157,47 +158,62 @@ int build_wmech(ohc_ctx *c) {
         auto key = [&](int r) { return m.kind[r] * 100 + m.rev[r] * 10 + m.fo_type[r]; };
         return key(a) < key(b);
     });
-    auto xi = [](int f, int q) { return ((size_t)(q / 64) * RI_NFIELD + f) * 64 + q % 64; };
+    auto xh = [](int f, int q) { return ((size_t)(q / 64) * RH_NFIELD + f) * 64 + q % 64; };
     auto xd = [](int f, int q) { return ((size_t)(q / 64) * RD_NFIELD + f) * 64 + q % 64; };
-    std::vector<int> ri((size_t)RI_NFIELD * nrp, 0);
+    auto fbits = [](double v) {
+        const float f = (float)v;
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return u;
+    };
+    std::vector<uint32_t> rh((size_t)RH_NFIELD * nrp, 0u);
     std::vector<double> rd((size_t)RD_NFIELD * nrp, 0.0), eff((size_t)std::max(nr, 1) * 64, 0.0);
     for (int q = 0; q < nr; ++q) {
         const int r = perm[q];
-        ri[xi(RI_NF, q)] = m.nf[r];
-        ri[xi(RI_NB, q)] = m.nb[r];
-        ri[xi(RI_REV, q)] = m.rev[r];
-        ri[xi(RI_KIND, q)] = m.kind[r];
-        ri[xi(RI_FOT, q)] = m.fo_type[r];
         double s = 0.0;
+        bool gen = false;  // a coefficient, order or their sum float does not hold exactly
+        auto put_f = [&](int f, double v) {
+            rh[xh(f, q)] = fbits(v);
+            gen = gen || (double)(float)v != v;
+        };
+        uint32_t fidx[kWP] = {0, 0, 0}, bidx[kWP] = {0, 0, 0};
         for (int i = 0; i < m.nf[r]; ++i) {
-            const int k = m.f_idx[r * P + i];
-            ri[xi(RI_FI + i, q)] = k;
+            fidx[i] = (uint32_t)m.f_idx[r * P + i];
             rd[xd(RD_FNU + i, q)] = m.f_nu[r * P + i];
             rd[xd(RD_FO + i, q)] = m.f_ord[r * P + i];
+            put_f(RH_FNU + i, m.f_nu[r * P + i]);
+            put_f(RH_FO + i, m.f_ord[r * P + i]);
             s -= m.f_nu[r * P + i];
         }
         for (int i = 0; i < m.nb[r]; ++i) {
-            const int k = m.b_idx[r * P + i];
-            ri[xi(RI_BI + i, q)] = k;
+            bidx[i] = (uint32_t)m.b_idx[r * P + i];
             rd[xd(RD_BNU + i, q)] = m.b_nu[r * P + i];
             rd[xd(RD_BO + i, q)] = m.b_ord[r * P + i];
+            put_f(RH_BNU + i, m.b_nu[r * P + i]);
+            put_f(RH_BO + i, m.b_ord[r * P + i]);
             s += m.b_nu[r * P + i];
         }
+        rd[xd(RD_SUMS, q)] = s;
+        put_f(RH_SUMS, s);
+        rh[xh(RH_HDR, q)] = (uint32_t)m.nf[r] | (uint32_t)m.nb[r] << 2 | (uint32_t)m.rev[r] << 4 |
+                            (uint32_t)m.kind[r] << 6 | (uint32_t)m.fo_type[r] << 9 |
+                            (gen ? kRhGen : 0u);
+        rh[xh(RH_IDX, q)] = fidx[0] | fidx[1] << 8 | fidx[2] << 16 | bidx[0] << 24;
+        rh[xh(RH_IDX2, q)] = bidx[1] | bidx[2] << 8;
         for (int i = 0; i < 3; ++i) {
             rd[xd(RD_A + i, q)] = m.arr[3 * r + i];
             rd[xd(RD_A0 + i, q)] = m.arr_lo[3 * r + i];
             rd[xd(RD_AR + i, q)] = m.arr_rev[3 * r + i];
         }
         for (int i = 0; i < 5; ++i) rd[xd(RD_FOP + i, q)] = m.fo[5 * r + i];
-        rd[xd(RD_SUMS, q)] = s;
         if (m.kind[r] != OHC_RATE_ARRHENIUS)
             for (int k = 0; k < ns; ++k) eff[(size_t)q * 64 + k] = m.eff[(size_t)r * ns + k];
     }
     // one device allocation, 256-byte aligned pieces
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
-    const size_t b_sp = al(sp.size() * 8), b_ri = al(ri.size() * 4), b_rd = al(rd.size() * 8),
+    const size_t b_sp = al(sp.size() * 8), b_rh = al(rh.size() * 4), b_rd = al(rd.size() * 8),
                  b_ef = al(eff.size() * 8);
-    const size_t total = b_sp + b_ri + b_rd + b_ef;
+    const size_t total = b_sp + b_rh + b_rd + b_ef;
     std::vector<unsigned char> h(total, 0);
     size_t off = 0;
     auto put = [&](const void *src, size_t bytes, size_t span) {
@@ -206,7 +222,7 @@ int build_wmech(ohc_ctx *c) {
         off += span;
         return o;
     };
-    const size_t o_sp = put(sp.data(), sp.size() * 8, b_sp), o_ri = put(ri.data(), ri.size() * 4, b_ri),
+    const size_t o_sp = put(sp.data(), sp.size() * 8, b_sp), o_rh = put(rh.data(), rh.size() * 4, b_rh),
                  o_rd = put(rd.data(), rd.size() * 8, b_rd), o_ef = put(eff.data(), eff.size() * 8, b_ef);
     HIPCHK(c, hipMalloc(&c->d_wm, total));
     HIPCHK(c, hipMemcpy(c->d_wm, h.data(), total, hipMemcpyHostToDevice));
@@ -215,7 +231,7 @@ int build_wmech(ohc_ctx *c) {
     c->wm.nr = nr;
     c->wm.nrp = nrp;
     c->wm.sp = (const double *)(d + o_sp);
-    c->wm.ri = (const int *)(d + o_ri);
+    c->wm.rh = (const unsigned *)(d + o_rh);
     c->wm.rd = (const double *)(d + o_rd);
     c->wm.eff = (const double *)(d + o_ef);
     return OHC_OK;

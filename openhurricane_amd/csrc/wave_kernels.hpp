@@ -31,10 +31,13 @@ namespace ohcd {
 
 constexpr int kWP = 3;  // distinct species per reaction side in the run-time tables
 
-// 1: the LU's pivot row reaches the other lanes by v_readlane pairs (no LDS
-// round trip and no barrier per elimination step); 0: through LDS.
-#ifndef OHC_LU_LANE_READS
-#define OHC_LU_LANE_READS 1
+// How the LU's pivot row reaches the other lanes: 1 by v_readlane pairs (two
+// vector ALU instructions per element, no LDS round trip), 2 by ds_bpermute
+// pairs (no vector ALU work, but an LDS round trip on every step's critical
+// path: 1.3x the LU time of 1 on the 32-species mechanism), 0 through an LDS
+// row buffer written by the pivot lane (a barrier per elimination step).
+#ifndef OHC_LU_BCAST
+#define OHC_LU_BCAST 1
 #endif
 static_assert(kWP == 3, "the reaction kernels multiply three participant powers");
 
@@ -59,17 +62,26 @@ enum {
 // Reaction tables are chunk-major: reaction r = 64 c + l (the chunk a
 // wavefront evaluates at once, l its lane) keeps field f at
 // [(c * NFIELD + f) * 64 + l].
-// reaction int fields
-enum { RI_NF, RI_NB, RI_FI, RI_BI = RI_FI + kWP, RI_REV = RI_BI + kWP, RI_KIND, RI_FOT, RI_NFIELD };
-// reaction double fields
+// Reaction 32-bit fields, read for every reaction:
+//   RH_HDR  nf | nb << 2 | rev << 4 | kind << 6 | fot << 9 | GEN << 12
+//   RH_IDX  fi0 | fi1 << 8 | fi2 << 16 | bi0 << 24;  RH_IDX2  bi1 | bi2 << 8
+//   RH_FNU.. RH_SUMS  stoichiometric coefficients, orders and their sum as
+//   float, exact for the small integers and halves of most mechanisms; a
+//   reaction with any other value has GEN set and reads the FP64 copies in
+//   the double table instead.
+enum { RH_HDR, RH_IDX, RH_IDX2, RH_FNU, RH_BNU = RH_FNU + kWP, RH_FO = RH_BNU + kWP,
+       RH_BO = RH_FO + kWP, RH_SUMS = RH_BO + kWP, RH_NFIELD };
+constexpr unsigned kRhGen = 1u << 12;
+// reaction double fields: the Arrhenius triple read for every reaction, then
+// what only some reactions read (fall-off, explicit reverse, GEN copies)
 enum {
+    RD_A,
+    RD_B,
+    RD_TA,
     RD_FNU,
     RD_BNU = RD_FNU + kWP,
     RD_FO = RD_BNU + kWP,
     RD_BO = RD_FO + kWP,
-    RD_A = RD_BO + kWP,
-    RD_B,
-    RD_TA,
     RD_A0,
     RD_B0,
     RD_TA0,
@@ -85,7 +97,7 @@ enum {
 struct WMech {
     int ns, nr, nrp;      // nrp = nr rounded up to a multiple of 64
     const double *sp;     // [SP_NFIELD][64]
-    const int *ri;        // [nrp / 64][RI_NFIELD][64]
+    const unsigned *rh;   // [nrp / 64][RH_NFIELD][64]
     const double *rd;     // [nrp / 64][RD_NFIELD][64]
     const double *eff;    // [nr][64] third-body efficiencies
 };
@@ -100,19 +112,22 @@ struct WaveParams {
 // the one vector offset, the chunk and field a scalar offset, so a table read
 // needs no vector address arithmetic.  Reads past a table return zero.
 struct RxnTab {
-    __amdgpu_buffer_rsrc_t ri, rd, ef;
+    __amdgpu_buffer_rsrc_t rh, rd, ef;
 };
 __device__ __forceinline__ RxnTab rxn_tab(const WMech &M) {
     constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit data, CDNA
     RxnTab t;
-    t.ri = __builtin_amdgcn_make_buffer_rsrc((void *)M.ri, 0, RI_NFIELD * M.nrp * 4, kRsrcWord3);
+    t.rh = __builtin_amdgcn_make_buffer_rsrc((void *)M.rh, 0, RH_NFIELD * M.nrp * 4, kRsrcWord3);
     t.rd = __builtin_amdgcn_make_buffer_rsrc((void *)M.rd, 0, RD_NFIELD * M.nrp * 8, kRsrcWord3);
     t.ef = __builtin_amdgcn_make_buffer_rsrc((void *)M.eff, 0, M.nr * 64 * 8, kRsrcWord3);
     return t;
 }
 // field f of reaction 64 c + lane (c wave-uniform)
-__device__ __forceinline__ int ldi(const RxnTab &t, int f, int c, int lane) {
-    return (int)__builtin_amdgcn_raw_buffer_load_b32(t.ri, lane * 4, (c * RI_NFIELD + f) * 256, 0);
+__device__ __forceinline__ unsigned ldu(const RxnTab &t, int f, int c, int lane) {
+    return __builtin_amdgcn_raw_buffer_load_b32(t.rh, lane * 4, (c * RH_NFIELD + f) * 256, 0);
+}
+__device__ __forceinline__ double ldf(const RxnTab &t, int f, int c, int lane) {
+    return (double)__builtin_bit_cast(float, ldu(t, f, c, lane));
 }
 __device__ __forceinline__ double ldd(const RxnTab &t, int f, int c, int lane) {
     return __builtin_bit_cast(
@@ -122,17 +137,15 @@ __device__ __forceinline__ double ldd(const RxnTab &t, int f, int c, int lane) {
 __device__ __forceinline__ double lde(const RxnTab &t, int r, int k) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(t.ef, r * 512, k * 8, 0));
 }
-// field f of reaction r for a wave-uniform r (scalar reads)
-__device__ __forceinline__ int ri_at(const WMech &M, int f, int r) {
-    return M.ri[((r >> 6) * RI_NFIELD + f) * 64 + (r & 63)];
-}
-__device__ __forceinline__ double rd_at(const WMech &M, int f, int r) {
-    return M.rd[((r >> 6) * RD_NFIELD + f) * 64 + (r & 63)];
-}
-
 __device__ __forceinline__ double rdlane(double v, int l) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+// lane (p4 / 4)'s v to every lane, through the LDS crossbar
+__device__ __forceinline__ double bperm(double v, int p4) {
+    const int lo = __builtin_amdgcn_ds_bpermute(p4, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(p4, __double2hiint(v));
     return __hiloint2double(hi, lo);
 }
 // Data-parallel-primitive move of a double within each row of 16 lanes
@@ -334,7 +347,7 @@ __device__ __forceinline__ void wthermo(const WMech &M, int k, const TFac &tf, d
 // partial derivatives of q_f / q_r by each participant's concentration.
 struct WRxn {
     double q, s, gam, dqf[kWP], dqr[kWP];
-    int nf, nb, fi[kWP], bi[kWP];
+    int nf, nb, rev, fi[kWP], bi[kWP];
     double fnu[kWP], bnu[kWP];
 };
 
@@ -347,18 +360,31 @@ __device__ __forceinline__ void wreaction(const WMech &M, int base, const TFac &
                                           int exact, WRxn &o) {
     const RxnTab tb = rxn_tab(M);
     const int lane = threadIdx.x, ch = base >> 6, r = base + lane;
-    const int nf = ldi(tb, RI_NF, ch, lane), nb = ldi(tb, RI_NB, ch, lane);
-    const int rev = ldi(tb, RI_REV, ch, lane), kind = ldi(tb, RI_KIND, ch, lane);
-    int fi[kWP], bi[kWP];
+    const unsigned hdr = ldu(tb, RH_HDR, ch, lane), ix = ldu(tb, RH_IDX, ch, lane),
+                   ix2 = ldu(tb, RH_IDX2, ch, lane);
+    const int nf = hdr & 3u, nb = (hdr >> 2) & 3u, rev = (hdr >> 4) & 3u, kind = (hdr >> 6) & 7u;
+    const int fi[kWP] = {(int)(ix & 255u), (int)((ix >> 8) & 255u), (int)((ix >> 16) & 255u)};
+    const int bi[kWP] = {(int)(ix >> 24), (int)(ix2 & 255u), (int)((ix2 >> 8) & 255u)};
     double fnu[kWP], bnu[kWP], fo[kWP], bo[kWP], cf[kWP], cb[kWP];
+    if (hdr & kRhGen) {
+#pragma unroll
+        for (int i = 0; i < kWP; ++i) {
+            fnu[i] = ldd(tb, RD_FNU + i, ch, lane);
+            bnu[i] = ldd(tb, RD_BNU + i, ch, lane);
+            fo[i] = ldd(tb, RD_FO + i, ch, lane);
+            bo[i] = ldd(tb, RD_BO + i, ch, lane);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kWP; ++i) {
+            fnu[i] = ldf(tb, RH_FNU + i, ch, lane);
+            bnu[i] = ldf(tb, RH_BNU + i, ch, lane);
+            fo[i] = ldf(tb, RH_FO + i, ch, lane);
+            bo[i] = ldf(tb, RH_BO + i, ch, lane);
+        }
+    }
 #pragma unroll
     for (int i = 0; i < kWP; ++i) {
-        fi[i] = ldi(tb, RI_FI + i, ch, lane);
-        bi[i] = ldi(tb, RI_BI + i, ch, lane);
-        fnu[i] = ldd(tb, RD_FNU + i, ch, lane);
-        bnu[i] = ldd(tb, RD_BNU + i, ch, lane);
-        fo[i] = ldd(tb, RD_FO + i, ch, lane);
-        bo[i] = ldd(tb, RD_BO + i, ch, lane);
         cf[i] = i < nf ? sc[fi[i]] : 1.0;
         cb[i] = i < nb ? sc[bi[i]] : 1.0;
     }
@@ -373,14 +399,14 @@ __device__ __forceinline__ void wreaction(const WMech &M, int base, const TFac &
     double kf, k0 = 0.0, kinf = 0.0, Pr = 0.0, F = 1.0;
     FoState fs{1.0, 0.0, 0.0};
     double fp[5];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) fp[i] = ldd(tb, RD_FOP + i, ch, lane);
-    const int fot = ldi(tb, RI_FOT, ch, lane);
+    const int fot = (hdr >> 9) & 7u;
     if (kind == 0) {
         kf = rarrh<FT>(A, b, Ta, tf);
     } else if (kind == 1) {
         kf = Mtb * rarrh<FT>(A, b, Ta, tf);
     } else {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) fp[i] = ldd(tb, RD_FOP + i, ch, lane);
         k0 = rarrh<FT>(ldd(tb, RD_A0, ch, lane), ldd(tb, RD_B0, ch, lane),
                        ldd(tb, RD_TA0, ch, lane), tf);
         kinf = rarrh<FT>(A, b, Ta, tf);
@@ -416,7 +442,7 @@ __device__ __forceinline__ void wreaction(const WMech &M, int base, const TFac &
         if constexpr (std::is_same_v<FT, float>) Kp = (double)__expf(-Kt);
         else Kp = exp(-Kt);
         if (bad(Kp)) Kp = kRootVeryLarge;
-        const double sS = ldd(tb, RD_SUMS, ch, lane);
+        const double sS = (hdr & kRhGen) ? ldd(tb, RD_SUMS, ch, lane) : ldf(tb, RH_SUMS, ch, lane);
         const bool zero_s = fabs(sS) < kVeryTiny;
         double Xs = 1.0, Kc;
         if (zero_s) {
@@ -459,6 +485,7 @@ __device__ __forceinline__ void wreaction(const WMech &M, int base, const TFac &
     o.q = q;
     o.nf = nf;
     o.nb = nb;
+    o.rev = rev;
 #pragma unroll
     for (int i = 0; i < kWP; ++i) {
         o.fi[i] = fi[i];
@@ -530,7 +557,7 @@ struct WaveLDS {
     double J[N * JS];
     double c[N], G[N], dG[N], Cp[N], H[N], dCp[N];  // species k in [k]
     double w[N];      // net production rates omega_k
-#if !OHC_LU_LANE_READS
+#if OHC_LU_BCAST == 0
     // pivot row of the current elimination step (element j at [2 j], b at
     // [2 N]): the stride keeps the writes single 64-bit stores straight from
     // the row's registers instead of packed 128-bit stores that need copies
@@ -582,7 +609,9 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
 #pragma unroll
                         for (int j = 0; j < kWP; ++j) {
                             if (j < o.nf) lds_add(&row[o.fi[j]], nu * o.dqf[j]);
-                            if (j < o.nb) lds_add(&row[o.bi[j]], -nu * o.dqr[j]);
+                            // (dq/dc of the products is zero for an
+                            // irreversible reaction: kr = 0)
+                            if (j < o.nb && o.rev != 0) lds_add(&row[o.bi[j]], -nu * o.dqr[j]);
                         }
                         lds_add(&row[ns], nu * o.s);
                     }
@@ -665,9 +694,9 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
 // compile time, so step k touches only the N - k - 1 columns right of the
 // pivot; FULL (n == N) drops the per-step bounds, so that the whole
 // factorisation is one straight-line block the compiler can schedule across
-// steps.  The pivot row reaches the other lanes through v_readlane pairs
-// (OHC_LU_LANE_READS = 0: through LDS); the pivot row and reciprocal of each
-// step are kept in lane k of two registers for the back substitution.
+// steps.  The pivot row reaches the other lanes as OHC_LU_BCAST selects; the
+// pivot row and reciprocal of each step are kept in lane k of two registers
+// for the back substitution.
 template <int N, bool FULL>
 __device__ __forceinline__ double wave_lu_solve_n(WaveLDS<N> &L, int n, double h, double b,
                                                   double diag) {
@@ -701,7 +730,7 @@ __device__ __forceinline__ double wave_lu_solve_n(WaveLDS<N> &L, int n, double h
             const unsigned mhi = wmax_u32(hi);
             const bool top = cand && hi == mhi;
             const unsigned lo = top ? (unsigned)__double2loint(av) : 0u;
-            const unsigned mlo = wmax_u32(lo);
+            const unsigned mlo = wmax_u32(lo);  // (every lane takes part in the reduction)
             const unsigned long long hit = __ballot(top && lo == mlo);
             const int p = hit ? (int)__ffsll((long long)hit) - 1 : 0;  // no hit only for NaN rows
             const double pk = rdlane(a[k], p);
@@ -714,7 +743,11 @@ __device__ __forceinline__ double wave_lu_solve_n(WaveLDS<N> &L, int n, double h
             a[k] = el ? l : a[k];
             // columns j >= n hold zeros in every row and stay zero, so the
             // update needs no per-column predicate
-#if OHC_LU_LANE_READS
+#if OHC_LU_BCAST == 2
+            const int p4 = p * 4;
+            sfor<k + 1, N>([&](auto j) { a[j] = fma(-l, bperm(a[j], p4), a[j]); });
+            b = fma(-l, bperm(b, p4), b);
+#elif OHC_LU_BCAST == 1
             sfor<k + 1, N>([&](auto j) { a[j] = fma(-l, rdlane(a[j], p), a[j]); });
             b = fma(-l, rdlane(b, p), b);
 #else
@@ -736,7 +769,9 @@ __device__ __forceinline__ double wave_lu_solve_n(WaveLDS<N> &L, int n, double h
         if (FULL || k < n) {
             const int pk = __builtin_amdgcn_readlane(permv, k);
             const double xk = rdlane(b, pk) * rdlane(dinvv, k);
-            b = step < k ? fma(-a[k], xk, b) : b;
+            // rows pivoted at step k or later have given their x already
+            // (or give it now), so their b may change freely
+            b = fma(-a[k], xk, b);
             x = lane == k ? xk : x;
         }
     });
