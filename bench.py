@@ -403,7 +403,8 @@ def h2air_run(args, n):
                     "ie_lane_h2air_1M_pmc.txt")
     roof["autotune"] = tune
     roof["props"] = props_roofline(props, T0, Y0, mech.ns, stream)
-    return mech, st, g, T0, Y0, dt, stream, ms, roof, sub
+    cons = None if args.no_consecutive else consecutive(cs, g, T0, Y0, dt, 3, stream)
+    return mech, st, g, T0, Y0, dt, stream, ms, roof, sub, cons
 
 
 def props_roofline(props, T, Y, ns, stream, reps=5):
@@ -434,10 +435,11 @@ def props_roofline(props, T, Y, ns, stream, reps=5):
 
 def h2air_leg(args):
     n = 1 << 20
-    mech, st, g, T0, Y0, dt, stream, ms, roof, _ = h2air_run(args, n)
+    mech, st, g, T0, Y0, dt, stream, ms, roof, _, cons = h2air_run(args, n)
     out = {"workload": "H2-air 9sp/19rxn, 1,048,576 cells, dt 1e-6 s, plain FP64 ImplicitEuler "
                        "(rtol 1e-6, atol 1e-10) + mu/kappa/D_k at the new state",
-           "value": n / (ms * 1e-3), "unit": "cells/s", "ms_per_step": ms, "roofline": roof}
+           "value": n / (ms * 1e-3), "unit": "cells/s", "ms_per_step": ms, "roofline": roof,
+           "consecutive_steps": cons}
     if not args.no_split:
         out["split_precision"] = split_precision(args, mech, g, T0, Y0, dt, stream)
         out["other_integrators"] = other_integrators(mech, g, T0, Y0, dt, stream)
@@ -448,7 +450,7 @@ def bench_h2air(args):
     from openhurricane_amd import _lib
 
     n = args.cells or (1 << 20)
-    mech, st, g, T0, Y0, dt, stream, ms, roof, sub = h2air_run(args, n)
+    mech, st, g, T0, Y0, dt, stream, ms, roof, sub, cons = h2air_run(args, n)
     split = None if args.no_split else split_precision(args, mech, g, T0, Y0, dt, stream)
     others = None if args.no_split else other_integrators(mech, g, T0, Y0, dt, stream)
     cpu = None
@@ -465,7 +467,7 @@ def bench_h2air(args):
                    "cells": n, "mechanism": "h2air", "integrator": "ImplicitEuler",
                    "parallelism": "single GPU"},
         "roofline": roof, "cpu_baseline": cpu, "split_precision": split,
-        "other_integrators": others,
+        "other_integrators": others, "consecutive_steps": cons,
     }
     print(json.dumps(line), flush=True)
 

@@ -688,8 +688,8 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
 }
 
 // Solve (diag I - h J) x = b with the Jacobian rows in LDS: lane i owns row i;
-// partial pivoting as SquareMatrix::LUDecomposeDPivoting (largest |pivot|,
-// lowest row among equals), forward substitution folded into the
+// partial pivoting as SquareMatrix::LUDecomposeDPivoting (largest |pivot| to
+// 2^-20 relative, lowest row among those), forward substitution folded into the
 // elimination.  Returns x_i in lane i.  The elimination steps are unrolled at
 // compile time, so step k touches only the N - k - 1 columns right of the
 // pivot; FULL (n == N) drops the per-step bounds, so that the whole
@@ -720,18 +720,18 @@ __device__ __forceinline__ double wave_lu_solve_n(WaveLDS<N> &L, int n, double h
     sfor<0, N>([&](auto kc) {
         constexpr int k = kc;
         if (FULL || k < n) {
-            // largest |a_ik| over the rows not yet used, lowest row among
-            // equals: the maximum of the high words of |a_ik|, then of the
-            // low words among the rows that reach it (two 32-bit reductions
-            // give the exact order of non-negative doubles)
+            // largest |a_ik| over the rows not yet used by one 32-bit
+            // reduction of the high words of |a_ik| (exponent and the top 20
+            // mantissa bits: the order of non-negative doubles to 2^-20
+            // relative), the lowest row among those that reach it.  Rows
+            // whose |a_ik| differ by less than that are equally good pivots;
+            // the Jacobian's own rounding (its atomic sums) already decides
+            // between them at that level.
             const double av = fabs(a[k]);
             const bool cand = step < 0 && !isnan(av);
             const unsigned hi = cand ? (unsigned)__double2hiint(av) : 0u;
             const unsigned mhi = wmax_u32(hi);
-            const bool top = cand && hi == mhi;
-            const unsigned lo = top ? (unsigned)__double2loint(av) : 0u;
-            const unsigned mlo = wmax_u32(lo);  // (every lane takes part in the reduction)
-            const unsigned long long hit = __ballot(top && lo == mlo);
+            const unsigned long long hit = __ballot(cand && hi == mhi);
             const int p = hit ? (int)__ffsll((long long)hit) - 1 : 0;  // no hit only for NaN rows
             const double pk = rdlane(a[k], p);
             const double rp = frcp(pk);
