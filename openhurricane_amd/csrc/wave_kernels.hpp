@@ -137,6 +137,10 @@ __device__ __forceinline__ double ldd(const RxnTab &t, int f, int c, int lane) {
 __device__ __forceinline__ double lde(const RxnTab &t, int r, int k) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(t.ef, r * 512, k * 8, 0));
 }
+// efficiency of species `lane` in reaction r (wave-uniform)
+__device__ __forceinline__ double lde_col(const RxnTab &t, int r, int lane) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(t.ef, lane * 8, r * 512, 0));
+}
 __device__ __forceinline__ double rdlane(double v, int l) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
@@ -234,7 +238,13 @@ __device__ __forceinline__ double rarrh_dkdT(double k, double b, double Ta, cons
 // derivative of Pr).  The published Troe and SRI forms as the reference
 // writes them, with log10 x = ln x / ln 10, 10^x = exp(x ln 10) and
 // a^X b^e = exp(X ln a + e ln b): one logarithm of Pr and of the centre /
-// SRI base shared by F and its derivatives.
+// SRI base shared by F and its derivatives.  Both forms are a weighted sum of
+// three exponentials in T, its logarithm and one final exponential:
+//   Troe  Fcent = (1 - alpha) exp(-T/T3) + alpha exp(-T/T1) [+ exp(-T2/T)]
+//   SRI   base  = exp(-T/c) + a exp(-b/T)
+// so every lane of a wavefront evaluates the same five transcendentals with
+// its own weights and exponents, and a chunk that mixes the forms pays for
+// one set, not for each form's.
 struct FoState {
     double F, gPr, gT;
 };
@@ -248,45 +258,43 @@ __device__ __forceinline__ FoState rfo_eval(int fot, const double *fp, const TFa
     const double rPr = frcp(Pr);
     const double lnPr = log(fmax(Pr, kTiny));
     const double logPr = lnPr * kRln10;
-    if (fot == 3) {  // SRI: d [a exp(-b/T) + exp(-T/c)]^X T^e, X = 1 / (1 + log10(Pr)^2)
-        const double a = fp[0], b = fp[1], c = fp[2], e = fp[4];
-        const double X = frcp(1.0 + logPr * logPr);
-        const double ea = exp(-b * tf.invT), ec = exp(-T / c);
-        const double base = a * ea + ec;
-        const double lnb = log(base);
-        o.F = fp[3] * exp(X * lnb + e * tf.lnT);
-        if (WITH_J) {
-            const double dXdPr = -X * X * 2.0 * logPr * rPr * kRln10;
-            o.gPr = dXdPr * lnb;
-            o.gT = e * tf.invT + X * (a * b * ea * tf.invT * tf.invT - ec / c) * frcp(base);
-        }
-        return o;
-    }
-    // Troe, three or four parameters: Fcent, c, n, d = 0.14
-    const double e1 = exp(-T / fp[1]), e2 = exp(-T / fp[2]);
-    double Fc = (1.0 - fp[0]) * e1 + fp[0] * e2;
-    double dFc = (1.0 - fp[0]) * e1 / (-fp[1]) + fp[0] * e2 / (-fp[2]);
-    if (fot == 2) {
-        const double e3 = exp(-fp[3] * tf.invT);
-        Fc += e3;
-        dFc += e3 * (fp[3] * tf.invT * tf.invT);
-    }
-    Fc = fmax(Fc, kTiny);
-    const double lnFc = log(Fc);
-    const double logFc = lnFc * kRln10;
+    const bool sri = fot == 3, t4 = fot == 2;
+    // exp(-T / p1) and exp(-T / p2) weighted w1, w2; exp(-p3 / T) weighted w3
+    const double p1 = sri ? fp[2] : fp[1], p2 = fp[2];
+    const double w1 = sri ? 1.0 : 1.0 - fp[0], w2 = sri ? 0.0 : fp[0];
+    const double p3 = sri ? fp[1] : (t4 ? fp[3] : 0.0), w3 = sri ? fp[0] : (t4 ? 1.0 : 0.0);
+    const double u1 = exp(-T / p1), u2 = exp(sri ? 0.0 : -T / p2), u3 = exp(-p3 * tf.invT);
+    double base = w1 * u1 + w2 * u2;
+    base += w3 * u3;
+    if (!sri) base = fmax(base, kTiny);
+    const double lnb = log(base);
+    // Troe: F = 10^(log10 Fcent / (1 + inv^2)), inv = (log10 Pr + c) / (n - d (log10 Pr + c))
+    const double logFc = lnb * kRln10;
     const double c = -0.4 - 0.67 * logFc, n = 0.75 - 1.27 * logFc, dd = 0.14;
     const double lpc = logPr + c;
     const double rnmd = frcp(n - dd * lpc);
     const double inv = lpc * rnmd;
     const double rq = frcp(1.0 + inv * inv);
-    o.F = exp(lnFc * rq);  // 10^(log10 Fcent / (1 + inv^2))
+    // SRI: F = d base^X T^e, X = 1 / (1 + log10(Pr)^2)
+    const double X = frcp(1.0 + logPr * logPr);
+    const double E = sri ? X * lnb + fp[4] * tf.lnT : lnb * rq;
+    o.F = (sri ? fp[3] : 1.0) * exp(E);
     if (WITH_J) {
-        // d(log10 F)/d(log10 Pr + c) and the T derivatives of log10 Fcent, c, n
+        // (d base / dT) / base
+        double db = w1 * u1 / (-p1) + w2 * u2 / (-p2);
+        db += w3 * u3 * (p3 * tf.invT * tf.invT);
+        db *= frcp(base);
+        // Troe: d(log10 F)/d(log10 Pr + c) and the T derivatives of log10
+        // Fcent, c, n
         const double k = -logFc * rq * rq * 2.0 * lpc * rnmd * rnmd;
-        const double dlF = dFc * frcp(Fc) * kRln10;
+        const double dlF = db * kRln10;
         const double dc = -0.67 * dlF, dn = -1.27 * dlF;
-        o.gPr = k * (1.0 + dd * inv) * rPr;
-        o.gT = kLn10 * (dlF * rq + k * (dc * (1.0 + dd * inv) - inv * dn));
+        const double gPr_t = k * (1.0 + dd * inv) * rPr;
+        const double gT_t = kLn10 * (dlF * rq + k * (dc * (1.0 + dd * inv) - inv * dn));
+        const double gPr_s = -X * X * 2.0 * logPr * rPr * kRln10 * lnb;
+        const double gT_s = fp[4] * tf.invT + X * db;
+        o.gPr = sri ? gPr_s : gPr_t;
+        o.gT = sri ? gT_s : gT_t;
     }
     return o;
 }
@@ -400,16 +408,19 @@ __device__ __forceinline__ void wreaction(const WMech &M, int base, const TFac &
     FoState fs{1.0, 0.0, 0.0};
     double fp[5];
     const int fot = (hdr >> 9) & 7u;
+    // one exponential for the three rate laws (divergent branches would
+    // each evaluate their own)
+    const double kA = rarrh<FT>(A, b, Ta, tf);
     if (kind == 0) {
-        kf = rarrh<FT>(A, b, Ta, tf);
+        kf = kA;
     } else if (kind == 1) {
-        kf = Mtb * rarrh<FT>(A, b, Ta, tf);
+        kf = Mtb * kA;
     } else {
 #pragma unroll
         for (int i = 0; i < 5; ++i) fp[i] = ldd(tb, RD_FOP + i, ch, lane);
         k0 = rarrh<FT>(ldd(tb, RD_A0, ch, lane), ldd(tb, RD_B0, ch, lane),
                        ldd(tb, RD_TA0, ch, lane), tf);
-        kinf = rarrh<FT>(A, b, Ta, tf);
+        kinf = kA;
         const double kinfz = kinf != 0 ? kinf : kVeryTiny;
         Pr = k0 / kinfz * Mtb;
         fs = rfo_eval<WITH_J>(fot, fp, tf, Pr);
@@ -621,31 +632,40 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
         }
         if (WITH_J) {
             // third-body / fall-off terms sigma_a gam_r eff_rj, column j in lane
-            // j: each lane updates only its own column, so plain read-modify-
-            // write suffices, one reaction of the chunk after another; the
-            // reaction's participants come from its lane's registers (the
-            // wavefront's LDS operations complete in program order, after the
-            // atomics above)
+            // j, one reaction of the chunk after another.  The reaction lane
+            // first folds gam into its participants' coefficients; the loop
+            // then needs per participant only its row and coefficient from
+            // that lane (v_readlane, wave-uniform) and one LDS atomic add per
+            // column lane, with no read-modify-write round trip.  The
+            // wavefront's LDS operations complete in program order, so every
+            // element receives the same sums in the same order as the
+            // participant rows above.
             unsigned long long tbm = __ballot(gam != 0.0);
+            if (tbm) {
+#pragma unroll
+                for (int i = 0; i < kWP; ++i) {
+                    o.fnu[i] = -o.fnu[i] * gam;
+                    o.bnu[i] = o.bnu[i] * gam;
+                }
+            }
+            const RxnTab tb = rxn_tab(M);
             while (tbm) {
                 const int sl = __ffsll((long long)tbm) - 1;
                 tbm &= tbm - 1ull;
-                const int rr = base + sl;
-                const double g = rdlane(gam, sl);
-                const double e = lane < ns ? M.eff[(size_t)rr * 64 + lane] : 0.0;
+                const double e = lde_col(tb, base + sl, lane);  // eff_rj, zero past ns
                 const int nf = __builtin_amdgcn_readlane(o.nf, sl),
                           nb = __builtin_amdgcn_readlane(o.nb, sl);
 #pragma unroll
                 for (int i = 0; i < kWP; ++i) {
-                    if (i < nf && lane < ns) {
+                    if (i < nf) {
                         const int k = __builtin_amdgcn_readlane(o.fi[i], sl);
-                        double *pj = &L.J[k * WaveLDS<N>::JS + lane];
-                        *pj += -rdlane(o.fnu[i], sl) * g * e;
+                        const double v = rdlane(o.fnu[i], sl) * e;
+                        if (lane < ns) lds_add(&L.J[k * WaveLDS<N>::JS + lane], v);
                     }
-                    if (i < nb && lane < ns) {
+                    if (i < nb) {
                         const int k = __builtin_amdgcn_readlane(o.bi[i], sl);
-                        double *pj = &L.J[k * WaveLDS<N>::JS + lane];
-                        *pj += rdlane(o.bnu[i], sl) * g * e;
+                        const double v = rdlane(o.bnu[i], sl) * e;
+                        if (lane < ns) lds_add(&L.J[k * WaveLDS<N>::JS + lane], v);
                     }
                 }
             }
