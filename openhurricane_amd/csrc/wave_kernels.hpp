@@ -181,14 +181,20 @@ __device__ __forceinline__ double wmax(double v) {
     v = fmax(v, dpp<kDppMirror>(v));
     return fmax(fmax(rdlane(v, 0), rdlane(v, 16)), fmax(rdlane(v, 32), rdlane(v, 48)));
 }
-__device__ __forceinline__ unsigned wmax_u32(unsigned v) {
+// Maximum over lanes 0 .. N-1 of v (v = 0 in lanes N .. 63): each row of 16
+// lanes reduces by DPP, then the rows by the row broadcasts of lanes 15 and
+// 31 (rows 1 and 3 take row 0 and 2's maximum, rows 2 and 3 then lane 31's),
+// which leave the maximum of rows 0 .. r in row r.
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+template <int N>
+__device__ __forceinline__ unsigned wmax_u32_rows(unsigned v) {
     v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, kDppXor1, 0xf, 0xf, false));
     v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, kDppXor2, 0xf, 0xf, false));
     v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, kDppHalfMirror, 0xf, 0xf, false));
     v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, kDppMirror, 0xf, 0xf, false));
-    const unsigned a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16),
-                   c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
-    return max(max(a, b), max(c, d));
+    if (N > 16) v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, kDppRowBcast15, 0xa, 0xf, false));
+    if (N > 32) v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, kDppRowBcast31, 0xc, 0xf, false));
+    return __builtin_amdgcn_readlane(v, (N - 1) | 15);
 }
 
 // LDS floating-point add (ds_add_f64), result unused.
@@ -723,20 +729,28 @@ __device__ __forceinline__ double wave_lu_solve_n(WaveLDS<N> &L, int n, double h
     const int lane = threadIdx.x;
     if (FULL) n = N;
     const bool act = lane < n;
-    // every lane loads a row that exists (lanes past the system read row 0
-    // and discard it), so the N loads issue back to back without a branch
-    // and a wait each
+    // every lane loads a row that exists (lanes past the system read row 0),
+    // so the N loads issue back to back without a branch and a wait each.
+    // With n = N the lanes past the system keep row 0's values: they are
+    // never pivots, so no other lane reads them, and their b and x are not
+    // used.
     const double *Jr = L.J + (act ? lane : 0) * WaveLDS<N>::JS;
     double a[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
+        // -h J, then + diag on the diagonal: two roundings, as the reference
+        // forms I - h J (ImplicitEuler.cpp:186-193), never one fused multiply-add
+#pragma clang fp contract(off)
         const double x = Jr[j];
-        const double v = act && (FULL || j < n) ? -h * x : 0.0;
-        a[j] = (act && j == lane) ? v + diag : v;
+        const double v = FULL || (act && j < n) ? -h * x : 0.0;
+        const bool dj = __builtin_amdgcn_inverse_ballot_w64(1ull << j);  // lane == j
+        a[j] = ((FULL || act) && dj) ? v + diag : v;
     }
-    int step = act ? -1 : N;  // elimination step at which this row became the pivot
-    int permv = 0;            // lane k: pivot row of step k
-    double dinvv = 0.0;       // lane k: 1 / pivot of step k
+    // rows not yet used as a pivot, as a lane mask: the per-row predicates
+    // of an elimination step are scalar mask operations
+    unsigned long long avail = __builtin_amdgcn_ballot_w64(act);
+    int permv = 0;       // lane k: pivot row of step k
+    double dinvv = 0.0;  // lane k: 1 / pivot of step k
     sfor<0, N>([&](auto kc) {
         constexpr int k = kc;
         if (FULL || k < n) {
@@ -748,19 +762,26 @@ __device__ __forceinline__ double wave_lu_solve_n(WaveLDS<N> &L, int n, double h
             // the Jacobian's own rounding (its atomic sums) already decides
             // between them at that level.
             const double av = fabs(a[k]);
-            const bool cand = step < 0 && !isnan(av);
-            const unsigned hi = cand ? (unsigned)__double2hiint(av) : 0u;
-            const unsigned mhi = wmax_u32(hi);
-            const unsigned long long hit = __ballot(cand && hi == mhi);
+            const unsigned long long candm = avail & __builtin_amdgcn_ballot_w64(!isnan(av));
+            const unsigned hi =
+                __builtin_amdgcn_inverse_ballot_w64(candm) ? (unsigned)__double2hiint(av) : 0u;
+            const unsigned mhi = wmax_u32_rows<N>(hi);
+            const unsigned long long hit = candm & __builtin_amdgcn_ballot_w64(hi == mhi);
             const int p = hit ? (int)__ffsll((long long)hit) - 1 : 0;  // no hit only for NaN rows
+            const unsigned long long pm = 1ull << p;
             const double pk = rdlane(a[k], p);
             const double rp = frcp(pk);
-            permv = lane == k ? p : permv;
-            dinvv = lane == k ? rp : dinvv;
-            const bool el = step < 0 && lane != p;
-            step = lane == p ? k : step;
-            const double l = el ? a[k] * rp : 0.0;
-            a[k] = el ? l : a[k];
+            // lane k: the step's pivot row and reciprocal, for the back
+            // substitution (register writes: a masked LDS store would split
+            // the straight-line block)
+            const bool lk = __builtin_amdgcn_inverse_ballot_w64(1ull << k);  // lane == k
+            permv = lk ? p : permv;
+            dinvv = lk ? rp : dinvv;
+            avail &= ~pm;
+            // (an eliminated row's a_ik is not needed again: the back
+            // substitution reads column k only in the rows pivoted before
+            // step k, and the multiplier goes straight into the update)
+            const double l = __builtin_amdgcn_inverse_ballot_w64(avail) ? a[k] * rp : 0.0;
             // columns j >= n hold zeros in every row and stay zero, so the
             // update needs no per-column predicate
 #if OHC_LU_BCAST == 2
@@ -771,15 +792,16 @@ __device__ __forceinline__ double wave_lu_solve_n(WaveLDS<N> &L, int n, double h
             sfor<k + 1, N>([&](auto j) { a[j] = fma(-l, rdlane(a[j], p), a[j]); });
             b = fma(-l, rdlane(b, p), b);
 #else
-            if (lane == p) {
+            if (__builtin_amdgcn_inverse_ballot_w64(pm)) {
                 sfor<k + 1, N>([&](auto j) { L.prow[2 * j] = a[j]; });
                 L.prow[2 * N] = b;
             }
-            __syncthreads();
+            // (the block is one wavefront, whose LDS accesses complete in
+            // program order: the reads below see the row, and the next
+            // step's row cannot overwrite them)
+            __builtin_amdgcn_wave_barrier();
             sfor<k + 1, N>([&](auto j) { a[j] = fma(-l, L.prow[2 * j], a[j]); });
             b = fma(-l, L.prow[2 * N], b);
-            // (the block is one wavefront, whose LDS accesses complete in
-            // program order: the next step's row cannot overwrite these reads)
 #endif
         }
     });
@@ -792,7 +814,7 @@ __device__ __forceinline__ double wave_lu_solve_n(WaveLDS<N> &L, int n, double h
             // rows pivoted at step k or later have given their x already
             // (or give it now), so their b may change freely
             b = fma(-a[k], xk, b);
-            x = lane == k ? xk : x;
+            x = __builtin_amdgcn_inverse_ballot_w64(1ull << k) ? xk : x;
         }
     });
     return x;
