@@ -720,9 +720,9 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
 // compile time, so step k touches only the N - k - 1 columns right of the
 // pivot; FULL (n == N) drops the per-step bounds, so that the whole
 // factorisation is one straight-line block the compiler can schedule across
-// steps.  The pivot row reaches the other lanes as OHC_LU_BCAST selects; the
-// pivot row and reciprocal of each step are kept in lane k of two registers
-// for the back substitution.
+// steps.  The pivot row reaches the other lanes as OHC_LU_BCAST selects; each
+// pivot row keeps its step and reciprocal in two registers for the back
+// substitution.
 template <int N, bool FULL>
 __device__ __forceinline__ double wave_lu_solve_n(WaveLDS<N> &L, int n, double h, double b,
                                                   double diag) {
@@ -749,8 +749,9 @@ __device__ __forceinline__ double wave_lu_solve_n(WaveLDS<N> &L, int n, double h
     // rows not yet used as a pivot, as a lane mask: the per-row predicates
     // of an elimination step are scalar mask operations
     unsigned long long avail = __builtin_amdgcn_ballot_w64(act);
-    int permv = 0;       // lane k: pivot row of step k
-    double dinvv = 0.0;  // lane k: 1 / pivot of step k
+    int stepv = N;       // the elimination step at which this row was the pivot
+    double rpv = 0.0;    // and 1 / its pivot
+    bool nan_col = false;  // a step found only NaN in its column
     sfor<0, N>([&](auto kc) {
         constexpr int k = kc;
         if (FULL || k < n) {
@@ -768,15 +769,16 @@ __device__ __forceinline__ double wave_lu_solve_n(WaveLDS<N> &L, int n, double h
             const unsigned mhi = wmax_u32_rows<N>(hi);
             const unsigned long long hit = candm & __builtin_amdgcn_ballot_w64(hi == mhi);
             const int p = hit ? (int)__ffsll((long long)hit) - 1 : 0;  // no hit only for NaN rows
+            nan_col = nan_col || hit == 0;
             const unsigned long long pm = 1ull << p;
             const double pk = rdlane(a[k], p);
             const double rp = frcp(pk);
-            // lane k: the step's pivot row and reciprocal, for the back
-            // substitution (register writes: a masked LDS store would split
+            // the pivot row keeps its step and reciprocal for the back
+            // substitution (register selects: a masked LDS store would split
             // the straight-line block)
-            const bool lk = __builtin_amdgcn_inverse_ballot_w64(1ull << k);  // lane == k
-            permv = lk ? p : permv;
-            dinvv = lk ? rp : dinvv;
+            const bool isp = __builtin_amdgcn_inverse_ballot_w64(pm);
+            stepv = isp ? k : stepv;
+            rpv = isp ? rp : rpv;
             avail &= ~pm;
             // (an eliminated row's a_ik is not needed again: the back
             // substitution reads column k only in the rows pivoted before
@@ -805,19 +807,32 @@ __device__ __forceinline__ double wave_lu_solve_n(WaveLDS<N> &L, int n, double h
 #endif
         }
     });
-    double x = 0.0;
+    double xo = 0.0;  // the unknown of this row's step, set at that step
     sfor<0, N>([&](auto kk) {
         constexpr int k = N - 1 - kk;
         if (FULL || k < n) {
-            const int pk = __builtin_amdgcn_readlane(permv, k);
-            const double xk = rdlane(b, pk) * rdlane(dinvv, k);
+            // x_k = b / pivot in the row pivoted at step k
+            const bool own = stepv == k;
+            const unsigned long long pkm = __builtin_amdgcn_ballot_w64(own);
+            const int pk = pkm ? (int)__ffsll((long long)pkm) - 1 : 0;
+            const double t = b * rpv;
+            const double xk = rdlane(t, pk);
             // rows pivoted at step k or later have given their x already
             // (or give it now), so their b may change freely
             b = fma(-a[k], xk, b);
-            x = __builtin_amdgcn_inverse_ballot_w64(1ull << k) ? xk : x;
+            xo = own ? t : xo;
         }
     });
-    return x;
+    // x_k to lane k: every pivot row sends its unknown to the lane of its
+    // step (one LDS permute); the lanes past the system, never pivots, keep
+    // their own 0
+    const int dst = (stepv < n ? stepv : lane) * 4;
+    const int lo = __builtin_amdgcn_ds_permute(dst, __double2loint(xo));
+    const int hi = __builtin_amdgcn_ds_permute(dst, __double2hiint(xo));
+    // a column whose remaining rows are all NaN leaves steps without a
+    // pivot row: the solution is NaN, as the reference's division by a NaN
+    // pivot makes it (the caller then rejects the Newton step)
+    return nan_col ? __builtin_nan("") : __hiloint2double(hi, lo);
 }
 template <int N>
 __device__ __forceinline__ double wave_lu_solve(WaveLDS<N> &L, int n, double h, double b,
