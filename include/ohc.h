@@ -37,6 +37,9 @@
  *   ohc_eval_rhs / ohc_eval_jacobian
  *       chemistrySource::DyDt / chemistrySource::jacobian
  *       (chemistrySource.cpp:863-907, :967-1055).
+ *   ohc_lu_solve
+ *       SquareMatrix::LUDecomposeDPivoting + LUBacksubstituteDPivoting
+ *       (SquareMatrix.hpp:185-238, LUDecompose.hpp:68-96).
  *   ohc_thermo_T_from_h
  *       thermoList::THa_p / THa_rho (src/speciesTransport/thermo/include/thermoList.inl:315-340,
  *       thermoList.cpp:504-533).
@@ -330,6 +333,19 @@ int ohc_eval_rhs(ohc_ctx *ctx, int64_t n, const double *y, int64_t ldy,
                  const double *p, double *f, int64_t ldf);
 int ohc_eval_jacobian(ohc_ctx *ctx, int64_t n, const double *y, int64_t ldy,
                       const double *p, double *f, int64_t ldf, double *J, int64_t ldJ);
+
+/* Batched dense solves A_m x_m = b_m of nmat n x n systems (1 <= n <= 64)
+ * with partial pivoting: SquareMatrix::LUDecomposeDPivoting followed by
+ * LUBacksubstituteDPivoting (src/Hurricane/dataStructure/containers/Matrices/
+ * include/SquareMatrix.hpp:185-238, src/Hurricane/mathematicalTools/
+ * LinearEquations/include/LUDecompose.hpp:68-96), the factorisation the
+ * integrators run on I - h J.  Entry (i,j) of system m at A[(i*n+j)*ldA + m]
+ * (the ohc_eval_jacobian layout), b_i and x_i at b[i*ldb + m], x[i*ldx + m].
+ * The pivot of each column is the largest |a_ik| to 2^-20 relative (lowest
+ * row among those); a column whose remaining rows are all NaN gives an
+ * all-NaN x.  Independent of the context's mechanism. */
+int ohc_lu_solve(ohc_ctx *ctx, int64_t nmat, int32_t n, const double *A, int64_t ldA,
+                 const double *b, int64_t ldb, double *x, int64_t ldx);
 
 /* thermoList::THa_p: T from absolute enthalpy h [J/kg] at pressure p and
  * composition Y, Newton from T_inout (tol 1e-4*T0, 100 iterations, limited to

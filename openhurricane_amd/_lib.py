@@ -90,6 +90,8 @@ _SIGS = [
     ("ohc_eval_rhs", C.c_int, [_V, C.c_int64, _V, C.c_int64, _V, _V, C.c_int64]),
     ("ohc_eval_jacobian", C.c_int, [_V, C.c_int64, _V, C.c_int64, _V, _V, C.c_int64, _V,
                                     C.c_int64]),
+    ("ohc_lu_solve", C.c_int, [_V, C.c_int64, C.c_int32, _V, C.c_int64, _V, C.c_int64, _V,
+                               C.c_int64]),
     ("ohc_thermo_T_from_h", C.c_int, [_V, C.c_int64, _V, _V, _V, C.c_int64, _V, _V]),
     ("ohc_mix_props", C.c_int, [_V, C.c_int64, _V, _V, _V, C.c_int64, _V, _V, _V, C.c_int64,
                                 _V, _V, _V]),

@@ -297,6 +297,19 @@ class ChemistrySource:
                                       _ld(f, n), self._p(J), _ld(J, n)), self._h)
         return f, J
 
+    # SquareMatrix::LUDecomposeDPivoting + LUBacksubstituteDPivoting
+    def lu_solve(self, A, b):
+        """x with A_m x_m = b_m for a batch of n x n systems: A is
+        (n * n, nmat) with entry (i, j) of system m at A[i * n + j, m] (the
+        eval_jacobian layout), b is (n, nmat)."""
+        n, nmat = b.shape
+        if A.shape != (n * n, nmat):
+            raise ValueError(f"A must be ({n * n}, {nmat})")
+        x = self._zeros(b.shape)
+        check(lib().ohc_lu_solve(self._h, nmat, n, self._p(A), _ld(A, nmat), self._p(b),
+                                 _ld(b, nmat), self._p(x), _ld(x, nmat)), self._h)
+        return x
+
     # thermoList::THa_p
     def T_from_h(self, h, p, Y, T0, want_flag=False):
         n = h.shape[0]

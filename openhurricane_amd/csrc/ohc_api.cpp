@@ -861,6 +861,30 @@ int ohc_eval_jacobian(ohc_ctx *c, int64_t n, const double *y, int64_t ldy, const
     return eval_common(c, n, y, ldy, p, f, ldf, J, ldJ);
 }
 
+int ohc_lu_solve(ohc_ctx *c, int64_t nmat, int32_t n, const double *A, int64_t ldA,
+                 const double *b, int64_t ldb, double *x, int64_t ldx) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (n < 1 || n > 64) return fail(c, OHC_ERR_ARG, "n must be in 1..64");
+    if (nmat < 0 || ldA < nmat || ldb < nmat || ldx < nmat) return fail(c, OHC_ERR_ARG, "bad sizes");
+    if (!A || !b || !x) return fail(c, OHC_ERR_ARG, "null array argument");
+    if (nmat == 0) return OHC_OK;
+    if (nmat > 0x7fffffffll) return fail(c, OHC_ERR_ARG, "nmat too large");
+    int64_t m = 0;
+    if (host_slices(c, nmat, sizeof(double) * ((size_t)n * n + 2 * (size_t)n), &m))
+        return run_slices(c, nmat, m, [&](int64_t s, int64_t k) {
+            return ohc_lu_solve(c, k, n, A + s, ldA, b + s, ldb, x + s, ldx);
+        });
+    Staging st(c);
+    void *dA, *db, *dx;
+    if ((rc = st.map_rows(A, (int64_t)n * n, nmat, ldA, sizeof(double), true, false, &dA))) return rc;
+    if ((rc = st.map_rows(b, n, nmat, ldb, sizeof(double), true, false, &db))) return rc;
+    if ((rc = st.map_rows(x, n, nmat, ldx, sizeof(double), false, true, &dx))) return rc;
+    ohcd::LuParams P{nmat, n, (const double *)dA, ldA, (const double *)db, ldb, (double *)dx, ldx};
+    HIPCHK(c, ohcd::wave_lu(P, c->stream));
+    return st.finish();
+}
+
 static int t_from_f(ohc_ctx *c, int64_t n, const double *h, const double *Y, int64_t ldY,
                     double *T, int32_t *iflag, int energy) {
     int rc = check_ctx(c);

@@ -12,6 +12,7 @@ namespace ohcd {
     hipError_t wave_eval_##n(const EvalParams &, const WMech &, int, hipStream_t);  \
     hipError_t wave_source_##n(const SourceParams &, const WMech &, hipStream_t);    \
     hipError_t wave_kim_##n(const KimParams &, const WMech &, int, hipStream_t);     \
+    hipError_t wave_lu_##n(const LuParams &, hipStream_t);                           \
     const void *wave_fn_##n();
 OHC_WAVE_DECL(12)
 OHC_WAVE_DECL(16)
@@ -30,11 +31,13 @@ struct Entry {
     hipError_t (*eval)(const EvalParams &, const WMech &, int, hipStream_t);
     hipError_t (*source)(const SourceParams &, const WMech &, hipStream_t);
     hipError_t (*kim)(const KimParams &, const WMech &, int, hipStream_t);
+    hipError_t (*lu)(const LuParams &, hipStream_t);
     const void *(*fn)();
 };
 
 #define OHC_WAVE(n) \
-    Entry{n, &wave_integrate_##n, &wave_eval_##n, &wave_source_##n, &wave_kim_##n, &wave_fn_##n}
+    Entry{n,           &wave_integrate_##n, &wave_eval_##n, &wave_source_##n, &wave_kim_##n, \
+          &wave_lu_##n, &wave_fn_##n}
 const Entry kWave[] = {OHC_WAVE(12), OHC_WAVE(16), OHC_WAVE(24), OHC_WAVE(33),
                        OHC_WAVE(40), OHC_WAVE(48), OHC_WAVE(55), OHC_WAVE(64)};
 #undef OHC_WAVE
@@ -69,6 +72,11 @@ hipError_t wave_source(const SourceParams &p, const WMech &m, hipStream_t s) {
 hipError_t wave_kim(const KimParams &p, const WMech &m, int fp32, hipStream_t s) {
     const Entry *e = pick(m.ns + 1);
     return e ? e->kim(p, m, fp32, s) : hipErrorInvalidValue;
+}
+
+hipError_t wave_lu(const LuParams &p, hipStream_t s) {
+    const Entry *e = pick(p.n);
+    return e ? e->lu(p, s) : hipErrorInvalidValue;
 }
 
 // ---- mixture thermo + transport for run-time tables ----------------------

@@ -1356,6 +1356,35 @@ struct KimParams {
     int exact;
 };
 
+// Batched dense solves A x = b (SquareMatrix::LUDecomposeDPivoting +
+// LUBacksubstituteDPivoting): entry (i, j) of system m at A[(i n + j) ldA + m]
+// (the layout of the Jacobian output), b_i / x_i at b[i ldb + m] / x[i ldx + m].
+struct LuParams {
+    int64_t nmat;
+    int n;
+    const double *A;
+    int64_t ldA;
+    const double *b;
+    int64_t ldb;
+    double *x;
+    int64_t ldx;
+};
+
+// One wavefront per system: rows into the LDS matrix, then the integrators'
+// LU with h = -1 and no diagonal shift, so that it factors A itself.
+template <int N>
+__global__ __launch_bounds__(64) void lu_wave_kernel(const LuParams P) {
+    __shared__ WaveLDS<N> L;
+    const int lane = threadIdx.x, n = P.n;
+    const int64_t m = blockIdx.x;
+    for (int e = lane; e < n * n; e += 64)
+        L.J[(e / n) * WaveLDS<N>::JS + e % n] = P.A[(int64_t)e * P.ldA + m];
+    __syncthreads();
+    const double b = lane < n ? P.b[(int64_t)lane * P.ldb + m] : 0.0;
+    const double x = wave_lu_solve<N>(L, n, -1.0, b, 0.0);
+    if (lane < n) P.x[(int64_t)lane * P.ldx + m] = x;
+}
+
 template <class DT>
 __device__ __forceinline__ void lds_add_t(DT *p, DT v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);

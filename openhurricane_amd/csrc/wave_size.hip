@@ -62,6 +62,11 @@ hipError_t OHC_CAT(wave_kim_, OHC_WAVE_N)(const KimParams &p, const WMech &m, in
     return hipGetLastError();
 }
 
+hipError_t OHC_CAT(wave_lu_, OHC_WAVE_N)(const LuParams &p, hipStream_t s) {
+    hipLaunchKernelGGL((lu_wave_kernel<OHC_WAVE_N>), dim3((unsigned)p.nmat), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
 const void *OHC_CAT(wave_fn_, OHC_WAVE_N)() {
     return (const void *)&ie_wave_kernel<OHC_WAVE_N, 0>;
 }
