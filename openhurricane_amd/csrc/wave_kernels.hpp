@@ -572,7 +572,12 @@ struct WaveLDS {
     // a CU's LDS.
     static constexpr int JS = N | 1;
     double J[N * JS];
-    double c[N], G[N], dG[N], Cp[N], H[N], dCp[N];  // species k in [k]
+    // species Cp and dCp/dT are read only by their own lane: above N = 40
+    // they stay in registers, so that the 55-row kernel fits six wavefronts
+    // in a CU's LDS
+    static constexpr bool kCpLds = N <= 40;
+    double c[N], G[N], dG[N], H[N];  // species k in [k]
+    double Cp[kCpLds ? N : 1], dCp[kCpLds ? N : 1];
     double w[N];      // net production rates omega_k
 #if OHC_LU_BCAST == 0
     // pivot row of the current elimination step (element j at [2 j], b at
@@ -585,19 +590,26 @@ struct WaveLDS {
 // dy/dt (and the Jacobian into L.J when WITH_J) at the wavefront-distributed
 // state yv (lane i holds y_i; lane ns holds T).  Returns f_i in lane i.
 template <int N, int FLOW, bool WITH_J, class F = double>
-__device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, double yv, int exact) {
+__device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, double yv, int exact,
+                                            double *cp_lane = nullptr) {
     const int lane = threadIdx.x, ns = M.ns;
     const double T = rdlane(yv, ns);
     const TFac tf = tfac(T);
+    double cpr = 0.0, dcpr = 0.0;  // this lane's Cp and dCp/dT when not in LDS
     if (lane < ns) {
         L.c[lane] = fmax(yv, 0.0);
         double G, dG, Cp, H, dCp;
         wthermo<F>(M, lane, tf, G, dG, Cp, H, dCp);
         L.G[lane] = G;
         L.dG[lane] = dG;
-        L.Cp[lane] = Cp;
         L.H[lane] = H;
-        L.dCp[lane] = dCp;
+        if constexpr (WaveLDS<N>::kCpLds) {
+            L.Cp[lane] = Cp;
+            L.dCp[lane] = dCp;
+        } else {
+            cpr = Cp;
+            dcpr = dCp;
+        }
     }
     if (WITH_J)
         for (int e = lane; e < N * WaveLDS<N>::JS; e += 64) L.J[e] = 0.0;
@@ -681,17 +693,23 @@ __device__ __forceinline__ double wave_eval(const WMech &M, WaveLDS<N> &L, doubl
     const double w = lane < ns ? L.w[lane] : 0.0;
     // temperature equation (chemistrySource.cpp:863-907, jacobian :993-1054)
     const bool sp = lane < ns;
+    double cpv = cpr, dcpv = dcpr;
+    if constexpr (WaveLDS<N>::kCpLds) {
+        cpv = sp ? L.Cp[lane] : 0.0;
+        dcpv = sp ? L.dCp[lane] : 0.0;
+    }
+    if (cp_lane) *cp_lane = cpv;
     double Hx = 0.0, Cx = 0.0;
     if (sp) {
         Hx = FLOW == 0 ? L.H[lane] : L.H[lane] - kRu * T;
-        Cx = FLOW == 0 ? L.Cp[lane] : L.Cp[lane] - kRu;
+        Cx = FLOW == 0 ? cpv : cpv - kRu;
     }
     const double cC = wsum(sp ? yv * Cx : 0.0);
     const double dTs = wsum(sp ? w * Hx : 0.0);
     const double rcC = frcp(cC);
     const double dT = dTs * rcC;
     if (WITH_J) {
-        const double dcC = wsum(sp ? yv * L.dCp[lane] : 0.0);
+        const double dcC = wsum(sp ? yv * dcpv : 0.0);
         double jt = 0.0;
         if (sp) {
             double v = 0.0;
@@ -1318,12 +1336,13 @@ __global__ __launch_bounds__(64) void source_wave_kernel(const SourceParams P, c
     const double W = sp ? M.sp[SP_W * 64 + lane] : 1.0;
     const double Yk = sp ? P.Y[lane * P.ldY + cell] : 0.0;
     const double y = sp ? rho * Yk / W : (lane == ns ? T : 0.0);
-    const double w = wave_eval<N, 0, true>(M, L, y, P.exact);
+    double cpl;
+    const double w = wave_eval<N, 0, true>(M, L, y, P.exact, &cpl);
     if (sp) P.R[lane * P.ldR + cell] = w * W;
     if (!P.D) return;
     const double Ri = kRu / W;
     const double ea = sp ? L.H[lane] / W - Ri * T : 0.0;
-    const double cv = wsum(sp ? (L.Cp[lane] / W - Ri) * Yk : 0.0);
+    const double cv = wsum(sp ? (cpl / W - Ri) * Yk : 0.0);
     const double b = 1.0 / (rho * cv);
     const double *row = L.J + lane * WaveLDS<N>::JS;
     if (!P.without_last) {
